@@ -1,0 +1,131 @@
+/* tiny_mc_gpu — C ABI of the MI355X photon-transport path.
+ *
+ * Drop-in for the reference's photon transport loop (computacionparalela/
+ * tiny_mc): where tiny_mc.c:43-49 seeds glibc rand() and calls
+ *     void photon(float* heats, float* heats_squared);      (photon.h:3)
+ * once per photon, a host program fills a tmc_params from the same params.h
+ * macros and makes one call that walks every photon on the GPU.
+ *
+ * Conventions
+ *  - Plain C types only; callers own every array.
+ *  - Functions return TMC_OK (0) or a negative tmc_status; none aborts or
+ *    prints.  tiny_mc_gpu_strerror() names a status.
+ *  - One implicit context per process on the HIP device current at the first
+ *    call (or the one given to tiny_mc_gpu_init); calls are serialised
+ *    internally and synchronous.
+ *  - Results are deterministic: the random stream is Philox4x32-10 keyed by
+ *    (seed, photon index) and the tallies are integers, so a run gives the
+ *    same bits on every launch shape and in every split of the photon range.
+ */
+#ifndef TINY_MC_GPU_H
+#define TINY_MC_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TMC_ABI_VERSION 1
+
+/* Largest shell count (LDS holds 16 bytes per shell per workgroup). */
+#define TMC_MAX_SHELLS 8192
+
+/* Fixed-point scales of the raw tallies (see tiny_mc_gpu_run_ex). */
+#define TMC_HEAT_FRAC_BITS 33
+#define TMC_HEAT2_FRAC_BITS 43
+
+typedef enum tmc_status {
+    TMC_OK = 0,
+    TMC_E_INVALID = -1, /* bad parameters or null pointer */
+    TMC_E_NODEVICE = -2, /* no HIP device usable */
+    TMC_E_HIP = -3, /* a HIP runtime call failed */
+    TMC_E_NOMEM = -4, /* device or host allocation failed */
+    TMC_E_RANGE = -5, /* photon index range overflows 64 bits */
+    TMC_E_UNSUPPORTED = -6 /* option not supported by this build */
+} tmc_status;
+
+/* Physics and size of one simulation; the fields are the macros of
+ * params.h:5-27 (SEED is the separate seed argument). */
+typedef struct tmc_params {
+    float mu_a; /* MU_A: absorption coefficient, 1/cm, > 0 */
+    float mu_s; /* MU_S: scattering coefficient, 1/cm, >= 0 */
+    uint32_t shells; /* SHELLS: 1..TMC_MAX_SHELLS; the last shell collects
+                               every deposit beyond the others (tiny_mc.c:66) */
+    float microns_per_shell; /* MICRONS_PER_SHELL: > 0 */
+    uint64_t photons; /* PHOTONS */
+} tmc_params;
+
+/* Direction sampler after each absorption (photon.c:34-43). */
+#define TMC_SAMPLER_ANALYTIC 0u /* cos(theta) = 2u-1, phi = 2 pi u' (default) */
+#define TMC_SAMPLER_REJECTION 1u /* the reference's rejection method */
+
+typedef struct tmc_options {
+    uint32_t sampler; /* TMC_SAMPLER_* */
+    uint32_t block; /* threads per workgroup, multiple of 64 in [64, 1024]; 0 = auto */
+    uint32_t grid; /* workgroups; 0 = auto (fills every CU) */
+    uint32_t chunk; /* photons a wave claims at a time, multiple of 64; 0 = auto */
+    uint64_t first_photon; /* index of the first photon of the stream to walk */
+    uint64_t max_interactions; /* stop a photon after this many absorptions; 0 = never */
+} tmc_options;
+
+typedef struct tmc_run_stats {
+    uint64_t photons; /* photons walked */
+    uint64_t interactions; /* absorption events (iterations of photon.c:20) */
+    uint64_t truncated; /* photons stopped by max_interactions */
+    double kernel_ms; /* walk kernel time, HIP events on the library stream */
+    double total_ms; /* whole call: clears, kernel, copies back */
+    uint32_t grid, block, chunk; /* launch shape used */
+    uint32_t sampler;
+} tmc_run_stats;
+
+/* Default options (analytic sampler, automatic launch shape, photon 0). */
+void tmc_options_default(tmc_options* opt);
+
+/* Walks p->photons photons of stream `seed` and writes, for every shell s,
+ *   heat[s]  = sum over deposits in s of (1 - albedo) * w
+ *   heat2[s] = sum over deposits in s of (1 - albedo)^2 * w^2
+ * i.e. the reference's heats / heats_squared (photon.c:30-31) after the
+ * whole loop (tiny_mc.c:47-49), in double.  Arrays have p->shells elements
+ * and are overwritten.  Replaces srand(SEED) + the photon() loop. */
+int tiny_mc_gpu_run(const tmc_params* p, uint64_t seed, double* heat, double* heat2);
+
+/* As tiny_mc_gpu_run with options, returning the exact integer tallies:
+ * tally has 4 * p->shells words; words [2s, 2s+1] are the (low, high) halves
+ * of the 128-bit sum of w * 2^TMC_HEAT_FRAC_BITS over deposits in shell s and
+ * words [2S + 2s, 2S + 2s + 1] those of (w*w) * 2^TMC_HEAT2_FRAC_BITS
+ * (w in binary32; both images are exact).  Tallies of disjoint photon ranges
+ * of one stream add as integers.  stats may be NULL. */
+int tiny_mc_gpu_run_ex(const tmc_params* p, uint64_t seed, const tmc_options* opt, uint64_t* tally,
+                       tmc_run_stats* stats);
+
+/* Converts raw tallies (as from tiny_mc_gpu_run_ex) into heat / heat2 in
+ * the reference's units, in double (host only, no device work). */
+int tiny_mc_gpu_tally_to_heat(const tmc_params* p, const uint64_t* tally, double* heat, double* heat2);
+
+/* Selects the HIP device for the implicit context (optional; default: the
+ * device current at the first call).  Fails if a context already exists on
+ * another device. */
+int tiny_mc_gpu_init(int device);
+
+/* Releases the implicit context and its device memory. */
+void tiny_mc_gpu_shutdown(void);
+
+/* Device name (gcnArchName) and compute-unit count of the context's device. */
+int tiny_mc_gpu_device_info(char* name, size_t cap, int* compute_units);
+
+/* Human-readable name of a tmc_status. */
+const char* tiny_mc_gpu_strerror(int status);
+
+/* Diagnostics used by the parity tests: the device Philox4x32-10 on n
+ * (counter, key) pairs (ctr: 4n words, key: 2n words, out: 4n words), and the
+ * device log / sin(2 pi a) / cos(2 pi a) of the walk on n inputs. */
+int tiny_mc_gpu_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out);
+int tiny_mc_gpu_eval_math(const float* in, uint32_t n, float* log_out, float* sin_out, float* cos_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

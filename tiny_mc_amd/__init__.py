@@ -1,0 +1,28 @@
+"""tiny_mc_amd: the MI355X photon-transport path of tiny_mc.
+
+The product is the C ABI in ``include/tiny_mc_gpu.h`` (``libtiny_mc_gpu.so``,
+HIP kernels for gfx950).  This package is its Python host binding, used by
+the tests and the benchmark, and mirrors the reference's interface: the
+params.h macros become ``Params`` fields and the photon() loop of
+tiny_mc.c:43-49 becomes one ``run`` call.
+"""
+from ._lib import (  # noqa: F401
+    HEAT2_FRAC_BITS,
+    HEAT_FRAC_BITS,
+    MAX_SHELLS,
+    SAMPLER_ANALYTIC,
+    SAMPLER_REJECTION,
+    Params,
+    RunStats,
+    TmcError,
+    device_info,
+    eval_math,
+    lib_path,
+    load,
+    philox,
+    run,
+    run_raw,
+    shutdown,
+    tally_to_heat,
+)
+from .table import format_table, table_rows  # noqa: F401

@@ -1,0 +1,221 @@
+"""ctypes binding of libtiny_mc_gpu.so (include/tiny_mc_gpu.h).
+
+The library is loaded from this package directory, where ``make -C
+tiny_mc_amd`` (or ``__graft_entry__.build()``) puts it.  There is no CPU
+fallback: every entry point that needs the GPU raises ``TmcError`` when the
+library is missing or the call fails.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HEAT_FRAC_BITS = 33
+HEAT2_FRAC_BITS = 43
+MAX_SHELLS = 8192
+SAMPLER_ANALYTIC = 0
+SAMPLER_REJECTION = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "libtiny_mc_gpu.so")
+
+_STATUS = {
+    0: "ok",
+    -1: "invalid argument",
+    -2: "no usable HIP device",
+    -3: "HIP runtime error",
+    -4: "out of memory",
+    -5: "photon index range overflows 64 bits",
+    -6: "unsupported option",
+}
+
+
+class TmcError(RuntimeError):
+    """A tiny_mc_gpu call failed; ``status`` holds the tmc_status code."""
+
+    def __init__(self, status, what):
+        super().__init__(f"{what}: {_STATUS.get(status, 'unknown status')} ({status})")
+        self.status = status
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("mu_a", ctypes.c_float),
+        ("mu_s", ctypes.c_float),
+        ("shells", ctypes.c_uint32),
+        ("microns_per_shell", ctypes.c_float),
+        ("photons", ctypes.c_uint64),
+    ]
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [
+        ("sampler", ctypes.c_uint32),
+        ("block", ctypes.c_uint32),
+        ("grid", ctypes.c_uint32),
+        ("chunk", ctypes.c_uint32),
+        ("first_photon", ctypes.c_uint64),
+        ("max_interactions", ctypes.c_uint64),
+    ]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [
+        ("photons", ctypes.c_uint64),
+        ("interactions", ctypes.c_uint64),
+        ("truncated", ctypes.c_uint64),
+        ("kernel_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("grid", ctypes.c_uint32),
+        ("block", ctypes.c_uint32),
+        ("chunk", ctypes.c_uint32),
+        ("sampler", ctypes.c_uint32),
+    ]
+
+
+@dataclass(frozen=True)
+class Params:
+    """One simulation: the params.h macros (params.h:5-27) as run-time values."""
+
+    mu_a: float = 2.0
+    mu_s: float = 20.0
+    shells: int = 101
+    microns_per_shell: float = 50.0
+    photons: int = 32768
+
+    def _c(self):
+        return _Params(self.mu_a, self.mu_s, self.shells, self.microns_per_shell, self.photons)
+
+
+@dataclass
+class RunStats:
+    photons: int
+    interactions: int
+    truncated: int
+    kernel_ms: float
+    total_ms: float
+    grid: int
+    block: int
+    chunk: int
+    sampler: int
+
+
+_lib = None
+
+
+def load():
+    """Loads libtiny_mc_gpu.so from the package directory (once)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(lib_path):
+        raise TmcError(-6, f"{lib_path} is not built (run make -C tiny_mc_amd)")
+    lib = ctypes.CDLL(lib_path)
+    P = ctypes.POINTER
+    u64p = P(ctypes.c_uint64)
+    dp = P(ctypes.c_double)
+    lib.tmc_options_default.argtypes = [P(_Options)]
+    lib.tmc_options_default.restype = None
+    lib.tiny_mc_gpu_run.argtypes = [P(_Params), ctypes.c_uint64, dp, dp]
+    lib.tiny_mc_gpu_run_ex.argtypes = [P(_Params), ctypes.c_uint64, P(_Options), u64p, P(_Stats)]
+    lib.tiny_mc_gpu_tally_to_heat.argtypes = [P(_Params), u64p, dp, dp]
+    lib.tiny_mc_gpu_init.argtypes = [ctypes.c_int]
+    lib.tiny_mc_gpu_shutdown.argtypes = []
+    lib.tiny_mc_gpu_shutdown.restype = None
+    lib.tiny_mc_gpu_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_int)]
+    lib.tiny_mc_gpu_strerror.argtypes = [ctypes.c_int]
+    lib.tiny_mc_gpu_strerror.restype = ctypes.c_char_p
+    u32p = P(ctypes.c_uint32)
+    fp = P(ctypes.c_float)
+    lib.tiny_mc_gpu_philox.argtypes = [u32p, u32p, ctypes.c_uint32, u32p]
+    lib.tiny_mc_gpu_eval_math.argtypes = [fp, ctypes.c_uint32, fp, fp, fp]
+    for name in ("tiny_mc_gpu_run", "tiny_mc_gpu_run_ex", "tiny_mc_gpu_tally_to_heat", "tiny_mc_gpu_init",
+                 "tiny_mc_gpu_device_info", "tiny_mc_gpu_philox", "tiny_mc_gpu_eval_math"):
+        getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise TmcError(rc, what)
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def run(params: Params, seed: int):
+    """heat, heat2 (float64 arrays of params.shells) in the reference's units."""
+    lib = load()
+    heat = np.zeros(params.shells, dtype=np.float64)
+    heat2 = np.zeros(params.shells, dtype=np.float64)
+    c = params._c()
+    _check(lib.tiny_mc_gpu_run(ctypes.byref(c), seed, _ptr(heat, ctypes.c_double), _ptr(heat2, ctypes.c_double)),
+           "tiny_mc_gpu_run")
+    return heat, heat2
+
+
+def run_raw(params: Params, seed: int, *, sampler=SAMPLER_ANALYTIC, first_photon=0, max_interactions=0,
+            block=0, grid=0, chunk=0):
+    """Integer tallies (uint64 array of 4*shells: 128-bit (lo, hi) sums) and RunStats."""
+    lib = load()
+    tally = np.zeros(4 * params.shells, dtype=np.uint64)
+    opt = _Options(sampler, block, grid, chunk, first_photon, max_interactions)
+    st = _Stats()
+    c = params._c()
+    _check(lib.tiny_mc_gpu_run_ex(ctypes.byref(c), seed, ctypes.byref(opt), _ptr(tally, ctypes.c_uint64),
+                                  ctypes.byref(st)), "tiny_mc_gpu_run_ex")
+    stats = RunStats(st.photons, st.interactions, st.truncated, st.kernel_ms, st.total_ms, st.grid, st.block,
+                     st.chunk, st.sampler)
+    return tally, stats
+
+
+def tally_to_heat(params: Params, tally):
+    """Host conversion of raw tallies to heat, heat2 (no device work)."""
+    lib = load()
+    tally = np.ascontiguousarray(tally, dtype=np.uint64)
+    heat = np.zeros(params.shells, dtype=np.float64)
+    heat2 = np.zeros(params.shells, dtype=np.float64)
+    c = params._c()
+    _check(lib.tiny_mc_gpu_tally_to_heat(ctypes.byref(c), _ptr(tally, ctypes.c_uint64),
+                                         _ptr(heat, ctypes.c_double), _ptr(heat2, ctypes.c_double)),
+           "tiny_mc_gpu_tally_to_heat")
+    return heat, heat2
+
+
+def philox(ctr, key):
+    """Device Philox4x32-10 on n counters (n x 4 uint32) and keys (n x 2 uint32)."""
+    lib = load()
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+    key = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1, 2)
+    assert ctr.shape[0] == key.shape[0]
+    out = np.zeros_like(ctr)
+    _check(lib.tiny_mc_gpu_philox(_ptr(ctr, ctypes.c_uint32), _ptr(key, ctypes.c_uint32), ctr.shape[0],
+                                  _ptr(out, ctypes.c_uint32)), "tiny_mc_gpu_philox")
+    return out
+
+
+def eval_math(x):
+    """Device log(x), sin(2 pi x), cos(2 pi x) of the walk."""
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    lg, sn, cs = (np.zeros_like(x) for _ in range(3))
+    fp = ctypes.c_float
+    _check(lib.tiny_mc_gpu_eval_math(_ptr(x, fp), x.size, _ptr(lg, fp), _ptr(sn, fp), _ptr(cs, fp)),
+           "tiny_mc_gpu_eval_math")
+    return lg, sn, cs
+
+
+def device_info():
+    lib = load()
+    buf = ctypes.create_string_buffer(128)
+    cus = ctypes.c_int()
+    _check(lib.tiny_mc_gpu_device_info(buf, len(buf), ctypes.byref(cus)), "tiny_mc_gpu_device_info")
+    return buf.value.decode(), cus.value
+
+
+def shutdown():
+    if _lib is not None:
+        _lib.tiny_mc_gpu_shutdown()

@@ -1,0 +1,433 @@
+/* C ABI of the photon-transport path (include/tiny_mc_gpu.h).
+ *
+ * Host side of one run: validate the params.h-style parameters, derive the
+ * binary32 constants exactly as photon.c:8-9 does, clear the device tallies
+ * and claim counter, launch the persistent walk kernel once, and copy the
+ * 128-bit integer tallies back.  The conversion to the reference's heat /
+ * heat2 units is done in double on the host.
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "../../include/tiny_mc_gpu.h"
+#include "tmc_internal.h"
+
+#define TMC_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+struct Context {
+    int device = -1;
+    int compute_units = 0;
+    char arch[64] = { 0 };
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    unsigned long long* d_tally = nullptr;
+    size_t tally_words = 0;
+    unsigned long long* d_counters = nullptr; /* 4 words */
+};
+
+std::mutex g_mutex;
+Context* g_ctx = nullptr;
+
+int destroy_context(Context* c)
+{
+    if (!c)
+        return TMC_OK;
+    (void)hipSetDevice(c->device);
+    if (c->d_tally)
+        (void)hipFree(c->d_tally);
+    if (c->d_counters)
+        (void)hipFree(c->d_counters);
+    if (c->ev_start)
+        (void)hipEventDestroy(c->ev_start);
+    if (c->ev_stop)
+        (void)hipEventDestroy(c->ev_stop);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TMC_OK;
+}
+
+/* Creates the context on `device` (-1: the current device). */
+int create_context(int device, Context** out)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return TMC_E_NODEVICE;
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess)
+            return TMC_E_NODEVICE;
+    }
+    if (device >= count)
+        return TMC_E_NODEVICE;
+    if (hipSetDevice(device) != hipSuccess)
+        return TMC_E_HIP;
+    Context* c = new (std::nothrow) Context;
+    if (!c)
+        return TMC_E_NOMEM;
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return TMC_E_HIP;
+    }
+    c->compute_units = prop.multiProcessorCount;
+    std::snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
+    /* default-priority stream (never a prioritised one) */
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess
+        || hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess
+        || hipMalloc((void**)&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess) {
+        destroy_context(c);
+        return TMC_E_HIP;
+    }
+    *out = c;
+    return TMC_OK;
+}
+
+int get_context(Context** out)
+{
+    if (!g_ctx) {
+        int rc = create_context(-1, &g_ctx);
+        if (rc != TMC_OK)
+            return rc;
+    }
+    if (hipSetDevice(g_ctx->device) != hipSuccess)
+        return TMC_E_HIP;
+    *out = g_ctx;
+    return TMC_OK;
+}
+
+int ensure_tally(Context* c, size_t words)
+{
+    if (c->tally_words >= words)
+        return TMC_OK;
+    if (c->d_tally)
+        (void)hipFree(c->d_tally);
+    c->d_tally = nullptr;
+    c->tally_words = 0;
+    if (hipMalloc((void**)&c->d_tally, words * sizeof(unsigned long long)) != hipSuccess)
+        return TMC_E_NOMEM;
+    c->tally_words = words;
+    return TMC_OK;
+}
+
+/* Parameter checks.  The reference has none (params.h:14 only warns that
+ * MU_A must be non-zero: with mu_a = 0 the weight never reaches the cutoff
+ * and photon() never returns). */
+int check_params(const tmc_params* p)
+{
+    if (!p)
+        return TMC_E_INVALID;
+    if (!std::isfinite(p->mu_a) || !std::isfinite(p->mu_s) || !std::isfinite(p->microns_per_shell))
+        return TMC_E_INVALID;
+    if (!(p->mu_a > 0.0f) || !(p->mu_s >= 0.0f) || !(p->microns_per_shell > 0.0f))
+        return TMC_E_INVALID;
+    if (p->shells < 1 || p->shells > TMC_MAX_SHELLS)
+        return TMC_E_INVALID;
+    /* binary32 albedo must stay below 1, or the weight never decays */
+    const volatile float sum = p->mu_s + p->mu_a;
+    const volatile float albedo = p->mu_s / sum;
+    if (!(albedo < 1.0f))
+        return TMC_E_INVALID;
+    const double spm = 1e4 / (double)p->microns_per_shell / (double)sum;
+    if (!std::isfinite((float)spm) || !((float)spm > 0.0f))
+        return TMC_E_INVALID;
+    return TMC_OK;
+}
+
+/* photon.c:8: albedo evaluated in float. */
+float derived_albedo(const tmc_params* p)
+{
+    const volatile float sum = p->mu_s + p->mu_a;
+    return p->mu_s / sum;
+}
+
+/* photon.c:9: 1e4 is a double literal, so the quotient is formed in double
+ * from the float sum mu_a + mu_s, then stored as float. */
+float derived_spm(const tmc_params* p)
+{
+    const volatile float sum = p->mu_a + p->mu_s;
+    return (float)(1e4 / (double)p->microns_per_shell / (double)sum);
+}
+
+double u128_to_double(uint64_t lo, uint64_t hi)
+{
+    /* hi * 2^64 + lo, with one rounding of the 128-bit value in the common
+     * case hi == 0 */
+    return std::ldexp((double)hi, 64) + (double)lo;
+}
+
+int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options* o, uint64_t* tally,
+               tmc_run_stats* st)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t S = p->shells;
+    if (o->sampler > TMC_SAMPLER_REJECTION)
+        return TMC_E_INVALID;
+    if (p->photons > UINT64_MAX - o->first_photon)
+        return TMC_E_RANGE;
+
+    uint32_t block = o->block ? o->block : 256u;
+    if (block < 64u || block > 1024u || (block % 64u) != 0u)
+        return TMC_E_INVALID;
+    uint32_t chunk = o->chunk ? o->chunk : 256u;
+    if (chunk < 64u || (chunk % 64u) != 0u || chunk > (1u << 24))
+        return TMC_E_INVALID;
+    uint32_t grid = o->grid;
+    if (grid == 0) {
+        int per_cu = 0;
+        if (tmc::walk_occupancy(o->sampler, block, S, &per_cu) != hipSuccess || per_cu <= 0)
+            return TMC_E_HIP;
+        grid = (uint32_t)per_cu * (uint32_t)c->compute_units;
+    }
+    /* Do not launch far more waves than photons to walk. */
+    const uint64_t waves_needed = (p->photons + 63u) / 64u;
+    const uint32_t waves_per_wg = block / 64u;
+    const uint64_t wg_needed = (waves_needed + waves_per_wg - 1) / waves_per_wg;
+    if (wg_needed < grid)
+        grid = (uint32_t)(wg_needed ? wg_needed : 1u);
+
+    int rc = ensure_tally(c, 4 * (size_t)S);
+    if (rc != TMC_OK)
+        return rc;
+
+    tmc::WalkArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.seed = seed;
+    a.first_photon = o->first_photon;
+    a.photons = p->photons;
+    a.albedo = derived_albedo(p);
+    a.spm = derived_spm(p);
+    a.shell_max = (float)(S - 1);
+    a.shells = S;
+    a.chunk = chunk;
+    /* Each wave swaps the workgroup's LDS sums out at least every
+     * flush_interval steps, so between two swaps of one entry at most
+     * waves_per_wg * 64 * flush_interval deposits of at most 2^43 (heat2
+     * image of w = 1) accumulate: keep that below 2^63. */
+    a.flush_interval = (1u << 14) / waves_per_wg;
+    uint64_t kcap = o->max_interactions ? o->max_interactions : 0xFFFFFFFFull;
+    if (kcap > 0xFFFFFFFFull)
+        kcap = 0xFFFFFFFFull;
+    a.kcap = (uint32_t)kcap;
+    a.counters = c->d_counters;
+    a.tally = c->d_tally;
+
+    float kernel_ms = 0.0f;
+    if (hipMemsetAsync(c->d_tally, 0, 4 * (size_t)S * sizeof(unsigned long long), c->stream) != hipSuccess
+        || hipMemsetAsync(c->d_counters, 0, 4 * sizeof(unsigned long long), c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    unsigned long long counters[4] = { 0, 0, 0, 0 };
+    if (p->photons > 0) {
+        if (hipEventRecord(c->ev_start, c->stream) != hipSuccess)
+            return TMC_E_HIP;
+        if (tmc::launch_walk(o->sampler, a, grid, block, c->stream) != hipSuccess)
+            return TMC_E_HIP;
+        if (hipEventRecord(c->ev_stop, c->stream) != hipSuccess)
+            return TMC_E_HIP;
+    }
+    if (hipMemcpyAsync(tally, c->d_tally, 4 * (size_t)S * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream)
+            != hipSuccess
+        || hipMemcpyAsync(counters, c->d_counters, sizeof counters, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    if (p->photons > 0 && hipEventElapsedTime(&kernel_ms, c->ev_start, c->ev_stop) != hipSuccess)
+        return TMC_E_HIP;
+    if (st) {
+        st->photons = p->photons;
+        st->interactions = counters[1];
+        st->truncated = counters[2];
+        st->kernel_ms = kernel_ms;
+        st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        st->grid = grid;
+        st->block = block;
+        st->chunk = chunk;
+        st->sampler = o->sampler;
+    }
+    return TMC_OK;
+}
+
+} // namespace
+
+TMC_EXPORT void tmc_options_default(tmc_options* opt)
+{
+    if (!opt)
+        return;
+    std::memset(opt, 0, sizeof *opt);
+    opt->sampler = TMC_SAMPLER_ANALYTIC;
+}
+
+TMC_EXPORT int tiny_mc_gpu_run_ex(const tmc_params* p, uint64_t seed, const tmc_options* opt, uint64_t* tally,
+                                  tmc_run_stats* stats)
+{
+    int rc = check_params(p);
+    if (rc != TMC_OK)
+        return rc;
+    if (!tally)
+        return TMC_E_INVALID;
+    tmc_options o;
+    if (opt)
+        o = *opt;
+    else
+        tmc_options_default(&o);
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Context* c = nullptr;
+    rc = get_context(&c);
+    if (rc != TMC_OK)
+        return rc;
+    return run_locked(c, p, seed, &o, tally, stats);
+}
+
+TMC_EXPORT int tiny_mc_gpu_tally_to_heat(const tmc_params* p, const uint64_t* tally, double* heat, double* heat2)
+{
+    int rc = check_params(p);
+    if (rc != TMC_OK)
+        return rc;
+    if (!tally || !heat || !heat2)
+        return TMC_E_INVALID;
+    const uint32_t S = p->shells;
+    /* (1 - albedo) in double from the parameters; the FP32 transport's
+     * albedo (photon.c:8) only drives the weight decay. */
+    const double albedo = (double)p->mu_s / ((double)p->mu_s + (double)p->mu_a);
+    const double oma = 1.0 - albedo;
+    const double s1 = std::ldexp(oma, -TMC_HEAT_FRAC_BITS);
+    const double s2 = std::ldexp(oma * oma, -TMC_HEAT2_FRAC_BITS);
+    for (uint32_t s = 0; s < S; ++s) {
+        heat[s] = s1 * u128_to_double(tally[2 * s], tally[2 * s + 1]);
+        heat2[s] = s2 * u128_to_double(tally[2 * S + 2 * s], tally[2 * S + 2 * s + 1]);
+    }
+    return TMC_OK;
+}
+
+TMC_EXPORT int tiny_mc_gpu_run(const tmc_params* p, uint64_t seed, double* heat, double* heat2)
+{
+    int rc = check_params(p);
+    if (rc != TMC_OK)
+        return rc;
+    if (!heat || !heat2)
+        return TMC_E_INVALID;
+    uint64_t* tally = new (std::nothrow) uint64_t[4 * (size_t)p->shells];
+    if (!tally)
+        return TMC_E_NOMEM;
+    rc = tiny_mc_gpu_run_ex(p, seed, nullptr, tally, nullptr);
+    if (rc == TMC_OK)
+        rc = tiny_mc_gpu_tally_to_heat(p, tally, heat, heat2);
+    delete[] tally;
+    return rc;
+}
+
+TMC_EXPORT int tiny_mc_gpu_init(int device)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (g_ctx) {
+        if (device >= 0 && device != g_ctx->device)
+            return TMC_E_INVALID;
+        return TMC_OK;
+    }
+    return create_context(device, &g_ctx);
+}
+
+TMC_EXPORT void tiny_mc_gpu_shutdown(void)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    destroy_context(g_ctx);
+    g_ctx = nullptr;
+}
+
+TMC_EXPORT int tiny_mc_gpu_device_info(char* name, size_t cap, int* compute_units)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Context* c = nullptr;
+    int rc = get_context(&c);
+    if (rc != TMC_OK)
+        return rc;
+    if (name && cap)
+        std::snprintf(name, cap, "%s", c->arch);
+    if (compute_units)
+        *compute_units = c->compute_units;
+    return TMC_OK;
+}
+
+TMC_EXPORT const char* tiny_mc_gpu_strerror(int status)
+{
+    switch (status) {
+    case TMC_OK: return "ok";
+    case TMC_E_INVALID: return "invalid argument";
+    case TMC_E_NODEVICE: return "no usable HIP device";
+    case TMC_E_HIP: return "HIP runtime error";
+    case TMC_E_NOMEM: return "out of memory";
+    case TMC_E_RANGE: return "photon index range overflows 64 bits";
+    case TMC_E_UNSUPPORTED: return "unsupported option";
+    default: return "unknown status";
+    }
+}
+
+namespace {
+
+/* Device buffers of a diagnostic call, freed on every path. */
+struct DevBufs {
+    void* p[4] = { nullptr, nullptr, nullptr, nullptr };
+    ~DevBufs()
+    {
+        for (void* q : p)
+            if (q)
+                (void)hipFree(q);
+    }
+};
+
+} // namespace
+
+TMC_EXPORT int tiny_mc_gpu_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out)
+{
+    if ((!ctr || !key || !out) && n)
+        return TMC_E_INVALID;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Context* c = nullptr;
+    int rc = get_context(&c);
+    if (rc != TMC_OK || n == 0)
+        return rc;
+    DevBufs b;
+    if (hipMalloc(&b.p[0], 16 * (size_t)n) != hipSuccess || hipMalloc(&b.p[1], 8 * (size_t)n) != hipSuccess
+        || hipMalloc(&b.p[2], 16 * (size_t)n) != hipSuccess)
+        return TMC_E_NOMEM;
+    if (hipMemcpyAsync(b.p[0], ctr, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess
+        || hipMemcpyAsync(b.p[1], key, 8 * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess
+        || tmc::launch_philox((const uint32_t*)b.p[0], (const uint32_t*)b.p[1], n, (uint32_t*)b.p[2], c->stream)
+            != hipSuccess
+        || hipMemcpyAsync(out, b.p[2], 16 * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipStreamSynchronize(c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    return TMC_OK;
+}
+
+TMC_EXPORT int tiny_mc_gpu_eval_math(const float* in, uint32_t n, float* log_out, float* sin_out, float* cos_out)
+{
+    if ((!in || !log_out || !sin_out || !cos_out) && n)
+        return TMC_E_INVALID;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Context* c = nullptr;
+    int rc = get_context(&c);
+    if (rc != TMC_OK || n == 0)
+        return rc;
+    DevBufs b;
+    const size_t bytes = 4 * (size_t)n;
+    for (int i = 0; i < 4; ++i)
+        if (hipMalloc(&b.p[i], bytes) != hipSuccess)
+            return TMC_E_NOMEM;
+    if (hipMemcpyAsync(b.p[0], in, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess
+        || tmc::launch_eval_math((const float*)b.p[0], n, (float*)b.p[1], (float*)b.p[2], (float*)b.p[3], c->stream)
+            != hipSuccess
+        || hipMemcpyAsync(log_out, b.p[1], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipMemcpyAsync(sin_out, b.p[2], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipMemcpyAsync(cos_out, b.p[3], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipStreamSynchronize(c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    return TMC_OK;
+}

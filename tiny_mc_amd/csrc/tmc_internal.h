@@ -1,0 +1,40 @@
+/* Internal interface between the C ABI (tmc_api.cpp) and the kernels
+ * (tmc_walk.hip).  Not installed; the public interface is
+ * include/tiny_mc_gpu.h. */
+#ifndef TMC_INTERNAL_H
+#define TMC_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tmc {
+
+/* Everything one launch of the walk kernel needs, passed by value. */
+struct WalkArgs {
+    uint64_t seed; /* Philox key */
+    uint64_t first_photon; /* photon index of claim offset 0 */
+    uint64_t photons; /* photons in this launch */
+    float albedo; /* mu_s / (mu_s + mu_a) in binary32, as photon.c:8 */
+    float spm; /* shells per mean free path, as photon.c:9 */
+    float shell_max; /* (float)(shells - 1) */
+    uint32_t shells;
+    uint32_t chunk; /* photons claimed per wave at a time (multiple of 64) */
+    uint32_t flush_interval; /* wave steps between LDS tally flushes */
+    uint32_t kcap; /* stop a photon after kcap absorptions */
+    uint32_t pad;
+    unsigned long long* counters; /* [0] claim offset, [1] interactions, [2] truncated */
+    unsigned long long* tally; /* 4*shells words of 128-bit sums (lo, hi) */
+};
+
+/* Launches the persistent walk kernel on `stream`.  lds_bytes = 16*shells. */
+hipError_t launch_walk(uint32_t sampler, const WalkArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
+
+/* Resident workgroups per CU for the walk kernel at this block size. */
+hipError_t walk_occupancy(uint32_t sampler, uint32_t block, uint32_t shells, int* wg_per_cu);
+
+hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out, hipStream_t stream);
+hipError_t launch_eval_math(const float* in, uint32_t n, float* lg, float* sn, float* cs, hipStream_t stream);
+
+} // namespace tmc
+
+#endif

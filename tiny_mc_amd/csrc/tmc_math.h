@@ -1,0 +1,94 @@
+/* FP32 elementary functions of the walk, written for gfx950.
+ *
+ * The walk needs log (step length, photon.c:21), sqrt (radius and direction,
+ * photon.c:26,42-43) and, for the analytic direction sampler, sin/cos of a
+ * uniform angle.  They are implemented here from fused multiply-adds and
+ * exact hardware primitives (frexp, round-to-nearest-even, IEEE sqrt and
+ * division), so that the host mirror that uses the same coefficients and
+ * operation order reproduces them bit for bit.
+ *
+ * log:     x = f * 2^e with f in [sqrt(1/2), sqrt(2)), g = f - 1 (exact by
+ *          Sterbenz), log x = e ln2 + g + g^2 Q(g); Q is a degree-8 discrete
+ *          minimax polynomial (gen_poly.py).  Max error 0.87 ulp over every
+ *          input the walk can produce (u = m 2^-24, m = 1..2^24).
+ * sincos:  2 pi a with a in [0, 1]: n = rint(4a), r = a - n/4 (exact),
+ *          sin(2 pi r) = r S(r^2) (degree-3 S), cos(2 pi r) = C(r^2)
+ *          (degree-4 C), then the quadrant rotation.  Max abs error 8.4e-8.
+ */
+#ifndef TMC_MATH_H
+#define TMC_MATH_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tmc {
+
+#define TMC_LN2_HI 0x1.62e400p-1f /* 17 significant bits: e * LN2_HI is exact */
+#define TMC_LN2_LO 0x1.7f7d1cp-20f
+#define TMC_SQRT_HALF 0x1.6a09e6p-1f
+
+__device__ __forceinline__ float log32(float x)
+{
+    int e = __builtin_amdgcn_frexp_expf(x);
+    float f = __builtin_amdgcn_frexp_mantf(x);
+    if (f < TMC_SQRT_HALF) {
+        f = f + f;
+        e -= 1;
+    }
+    const float g = f - 1.0f;
+    float q = -0x1.38301cp-4f;
+    q = __builtin_fmaf(q, g, 0x1.084864p-3f);
+    q = __builtin_fmaf(q, g, -0x1.0f3766p-3f);
+    q = __builtin_fmaf(q, g, 0x1.22719cp-3f);
+    q = __builtin_fmaf(q, g, -0x1.54276ep-3f);
+    q = __builtin_fmaf(q, g, 0x1.99a3f2p-3f);
+    q = __builtin_fmaf(q, g, -0x1.000426p-2f);
+    q = __builtin_fmaf(q, g, 0x1.555550p-2f);
+    q = __builtin_fmaf(q, g, -0x1.fffff8p-2f);
+    const float l = __builtin_fmaf(g * g, q, g);
+    const float ef = (float)e;
+    return __builtin_fmaf(ef, TMC_LN2_HI, __builtin_fmaf(ef, TMC_LN2_LO, l));
+}
+
+__device__ __forceinline__ void sincos2pi32(float a, float* s_out, float* c_out)
+{
+    const float n = __builtin_rintf(a * 4.0f);
+    const float r = __builtin_fmaf(n, -0.25f, a);
+    const float r2 = r * r;
+    float ps = -0x1.2d9302p+6f;
+    ps = __builtin_fmaf(ps, r2, 0x1.465e92p+6f);
+    ps = __builtin_fmaf(ps, r2, -0x1.4abbbap+5f);
+    ps = __builtin_fmaf(ps, r2, 0x1.921fb6p+2f);
+    const float sn = r * ps;
+    float pc = 0x1.d9d57ep+5f;
+    pc = __builtin_fmaf(pc, r2, -0x1.55c5e0p+6f);
+    pc = __builtin_fmaf(pc, r2, 0x1.03c1dep+6f);
+    pc = __builtin_fmaf(pc, r2, -0x1.3bd3ccp+4f);
+    pc = __builtin_fmaf(pc, r2, 1.0f);
+    const int q = (int)n;
+    const bool swap = (q & 1) != 0;
+    const float s0 = swap ? pc : sn;
+    const float c0 = swap ? sn : pc;
+    /* quadrant 0: ( s,  c)  1: ( c, -s)  2: (-s, -c)  3: (-c,  s) */
+    *s_out = (q & 2) ? -s0 : s0;
+    *c_out = ((q + 1) & 2) ? -c0 : c0;
+}
+
+/* Correctly rounded square root (IEEE), the compiler's default lowering of
+ * sqrtf under -fhip-fp32-correctly-rounded-divide-sqrt. */
+__device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
+
+/* Fixed-point images of one deposit: w * 2^33 and (w*w) * 2^43, truncated.
+ * Exact for every w in [2^-10, 1]; the walk deposits only weights in
+ * [0.001, 1] (weights below the 0.001 cutoff are rouletted before the next
+ * deposit, photon.c:45-49), so the sums are the exact sums of the FP32
+ * deposits. */
+#define TMC_HEAT_FRAC_BITS 33
+#define TMC_HEAT2_FRAC_BITS 43
+
+__device__ __forceinline__ uint64_t fx_heat(float w) { return (uint64_t)(w * 0x1p33f); }
+__device__ __forceinline__ uint64_t fx_heat2(float w) { return (uint64_t)((w * w) * 0x1p43f); }
+
+} // namespace tmc
+
+#endif

@@ -1,0 +1,66 @@
+/* Philox4x32-10 counter-based generator, for host C/C++ and HIP device code.
+ *
+ * Replaces the reference's glibc rand() stream (tiny_mc.c:43 srand,
+ * photon.c:21,37,38,46 rand) with a generator that is a pure function of
+ * (counter, key): every photon owns the stream keyed by the run seed with
+ * counter (interaction k, block b, photon_lo, photon_hi), so lanes hold no
+ * generator state and results do not depend on which lane walks which photon.
+ *
+ * Algorithm: Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as
+ * 1, 2, 3", SC'11 — 4 words of 32 bits, 10 rounds, multipliers 0xD2511F53 and
+ * 0xCD9E8D57, Weyl key increments 0x9E3779B9 and 0xBB67AE85.
+ *
+ * On gfx950 each round is two v_mad_u64_u32 (hi and lo of a 32x32 product in
+ * one instruction) plus two three-input XORs; the key schedule depends only on
+ * the seed, so within a wave it lives in scalar registers.
+ */
+#ifndef TMC_PHILOX_H
+#define TMC_PHILOX_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define TMC_HD __host__ __device__ __forceinline__
+#else
+#define TMC_HD static inline
+#endif
+
+#define TMC_PHILOX_M0 0xD2511F53u
+#define TMC_PHILOX_M1 0xCD9E8D57u
+#define TMC_PHILOX_W0 0x9E3779B9u
+#define TMC_PHILOX_W1 0xBB67AE85u
+
+typedef struct tmc_u32x4 {
+    uint32_t x0, x1, x2, x3;
+} tmc_u32x4;
+
+TMC_HD tmc_u32x4 tmc_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
+{
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = (uint64_t)c0 * TMC_PHILOX_M0;
+        const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += TMC_PHILOX_W0;
+        k1 += TMC_PHILOX_W1;
+    }
+    tmc_u32x4 r = { c0, c1, c2, c3 };
+    return r;
+}
+
+/* 32 random bits -> uniform in (0, 1]: ((r >> 8) + 1) * 2^-24, exact in
+ * binary32.  The reference maps rand() to [0, 1] (photon.c:21); excluding 0
+ * keeps -log(xi) finite, which the reference does not guarantee. */
+TMC_HD float tmc_u01(uint32_t r)
+{
+    return (float)((r >> 8) + 1u) * 0x1p-24f;
+}
+
+#endif

@@ -1,0 +1,223 @@
+"""Benchmark of the MI355X photon-transport path (one GPU).
+
+A step is one launch of the walk over 2^30 photons at default physics
+(mu_a = 2/cm, mu_s = 20/cm, 101 shells of 50 um, weight cutoff 0.001,
+roulette 0.1), through the C ABI call tiny_mc_gpu_run_ex: clear the tallies,
+run the persistent kernel, copy the 128-bit tallies back.  There is no input
+data: photons are generated from (seed, photon index) on the device.
+
+Prints one JSON line: whole-job photons/s, the roofline of the walk kernel
+(algorithmic FP32 FLOPs of the reference algorithm, 40.5 per absorption
+event counted from photon.c:21-48, over the kernel's average duration
+measured with HIP events on the library's stream), and the reference CPU
+loop timed on this machine's host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons N]
+                    [--sampler analytic|rejection] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "photons/s at default physics; sweep photons/s + p50/p99 latency at fixed load"
+FLOP_PER_INTERACTION = 40.5  # FP32 FLOPs of photon.c:21-48 per absorption event
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "walk_f32_default_2e30_traffic.txt")
+REF_BENCH = os.path.join(ROOT, "oracle", "_ref", "tiny_mc_bench")
+
+
+def cpu_baseline(max_procs=16, photons_per_proc=1 << 18):
+    """The reference loop (photon() + glibc rand(), unmodified sources built
+    by oracle/Makefile) on the host cores: one process per core, distinct
+    seeds (threads would serialise on rand()'s lock).  Σ photons / wall."""
+    procs = max(1, min(max_procs, len(os.sched_getaffinity(0))))
+    if os.path.exists(REF_BENCH):
+        cmd, kind = [REF_BENCH], "reference"
+    else:
+        # the oracle's byte-identical restatement of the same loop
+        cmd, kind = [sys.executable, os.path.abspath(__file__), "--cpu-port-worker"], "port"
+    env_key = "TMC_PHOTONS"
+    t0 = time.perf_counter()
+    running = []
+    for i in range(procs):
+        env = dict(os.environ, TMC_SEED=str(i + 1), **{env_key: str(photons_per_proc)})
+        running.append(subprocess.Popen(cmd, env=env, stdout=subprocess.PIPE, text=True))
+    rates = []
+    for p in running:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            return None
+        m = re.search(r"^# ([0-9.]+) K photons per second", out, re.M)
+        if m:
+            rates.append(float(m.group(1)) * 1e3)
+    wall = time.perf_counter() - t0
+    return {
+        "value": procs * photons_per_proc / wall,
+        "unit": "photons/s",
+        "cores": procs,
+        "kind": kind,
+        "single_core_value": sum(rates) / len(rates) if rates else None,
+        "sample": f"{procs} processes x {photons_per_proc} photons, default physics, seeds 1..{procs}; "
+                  + ("unmodified reference tiny_mc.c/photon.c/wtime.c built -std=c11 -O3 -march=x86-64-v3"
+                     if kind == "reference" else "oracle restatement of the reference loop (oracle/liboracle.so)"),
+    }
+
+
+def cpu_port_worker():
+    """One process of the fallback CPU baseline: the oracle's restatement of
+    the reference loop (glibc rand(), FP32), timed like tiny_mc.c:45-56."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    orc = oracle_lib.load()
+    photons = int(os.environ["TMC_PHOTONS"])
+    t0 = time.perf_counter()
+    orc.restate(orc.params(), int(os.environ["TMC_SEED"]), photons)
+    dt = time.perf_counter() - t0
+    print(f"# {dt:f} seconds\n# {1e-3 * photons / dt:f} K photons per second")
+
+
+def copy_calibration(nbytes=1 << 30):
+    """A plain device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of the walk kernel (run under rocprofv3 --pmc)."""
+    import torch
+
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
+
+
+def rocpd_summary(db_path):
+    """Text summary of a rocprofv3 output database (rocpd SQLite): per kernel
+    the dispatch count and duration statistics, and per kernel and counter
+    the per-dispatch value (summed over hardware instances) averaged over
+    dispatches."""
+    import sqlite3
+
+    con = sqlite3.connect(db_path)
+    lines = [f"# rocpd summary of {os.path.basename(db_path)}"]
+    rows = con.execute("select name, count(*), avg(duration), min(duration), max(duration) from kernels "
+                       "group by name order by sum(duration) desc").fetchall()
+    if rows:
+        lines.append("kernel | calls | avg_ns | min_ns | max_ns")
+        lines += [f"{n} | {c} | {a:.0f} | {lo} | {hi}" for n, c, a, lo, hi in rows]
+    rows = con.execute("select kernel_name, counter_name, dispatch_id, sum(value) from counters_collection "
+                       "group by kernel_name, counter_name, dispatch_id").fetchall()
+    if rows:
+        acc = {}
+        for kname, cname, _, v in rows:
+            acc.setdefault((kname, cname), []).append(v)
+        lines.append("kernel | counter | dispatches | avg_per_dispatch")
+        lines += [f"{k} | {c} | {len(v)} | {sum(v) / len(v):.6g}" for (k, c), v in sorted(acc.items())]
+    return "\n".join(lines) + "\n"
+
+
+def read_traffic():
+    """HBM bytes per walk launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--photons", type=int, default=1 << 30)
+    ap.add_argument("--sampler", choices=["analytic", "rejection"], default="analytic")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-port-worker", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--rocpd-summary", metavar="DB", help="print a text summary of a rocprofv3 results database")
+    args = ap.parse_args()
+    if args.rocpd_summary:
+        return print(rocpd_summary(args.rocpd_summary), end="")
+    if args.cpu_port_worker:
+        return cpu_port_worker()
+    if args.copy_calibration:
+        return copy_calibration()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        sys.exit("bench.py: this path runs on one GPU only; --gpus must be 1 "
+                 "(photon ranges shard across GPUs with one 2*SHELLS-word sum, not measured here)")
+
+    import tiny_mc_amd
+
+    sampler = tiny_mc_amd.SAMPLER_ANALYTIC if args.sampler == "analytic" else tiny_mc_amd.SAMPLER_REJECTION
+    params = tiny_mc_amd.Params(mu_a=2.0, mu_s=20.0, shells=101, microns_per_shell=50.0, photons=args.photons)
+    name, cus = tiny_mc_amd.device_info()
+
+    for i in range(args.warmup):
+        tiny_mc_amd.run_raw(params, 1000 + i, sampler=sampler)
+    kernel_ms, interactions = [], 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, st = tiny_mc_amd.run_raw(params, i + 1, sampler=sampler)
+        kernel_ms.append(st.kernel_ms)
+        interactions += st.interactions
+    elapsed = time.perf_counter() - t0  # every call is synchronous
+
+    photons = args.steps * args.photons
+    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) * 1e-3
+    flop_per_launch = FLOP_PER_INTERACTION * interactions / args.steps
+    achieved = flop_per_launch / avg_kernel_s / 1e12
+    result = {
+        "metric": METRIC,
+        "value": photons / elapsed,
+        "unit": "photons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain walk at scale: {args.photons} photons, default physics (mu_a=2/cm, mu_s=20/cm, "
+                        "101 shells x 50 um, cutoff 0.001, roulette 0.1), one launch per step",
+            "photons_per_step": args.photons,
+            "shells": 101,
+            "sampler": args.sampler,
+            "transport": "fp32",
+            "tallies": "int64 fixed point (128-bit global sums)",
+            "device": name,
+            "compute_units": cus,
+        },
+        "interactions_per_s": interactions / elapsed,
+        "interactions_per_photon": interactions / photons,
+        "kernel_ms_avg": avg_kernel_s * 1e3,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": read_traffic(),
+            "kernel": "walk_f32",
+            "flop_per_launch": flop_per_launch,
+        },
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
+    }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -730,3 +730,64 @@ ORC_EXPORT void orc_eval_math32(const float* in, uint64_t n, float* lg, float* s
         orc_sincos2pi32(in[i], &sn[i], &cs[i]);
     }
 }
+
+/* The reference loop of orc_restate (glibc rand(), FP32 transport in the
+ * reference's operation order) with each FP32 deposit (1 - a) * w and its
+ * square summed in long double instead of accumulated in float
+ * (tiny_mc.c:26-27, photon.c:30-31): separates the reference's physics from
+ * the rounding of its float tallies. */
+typedef struct exact_sink {
+    long double* heat;
+    long double* heat2;
+    float one_minus_albedo;
+} exact_sink;
+
+#define W_FN restate_exact_photon
+#define W_REAL float
+#define W_SRC glibc_src
+#define W_SINK exact_sink
+#define W_U_STEP(s, k) glibc_u()
+#define W_U_DIR(s, k, a, i) glibc_u()
+#define W_U_ROUL(s, k) glibc_u()
+#define W_LOG logf
+#define W_SQRT sqrtf
+#define W_SINCOS2PI(a, s, c) (s = 0.0f, c = 0.0f)
+#define W_DEPOSIT(sink, shell, wt)                                                                     \
+    do {                                                                                             \
+        const float d_ = (sink)->one_minus_albedo * (wt);                                            \
+        const float d2_ = (sink)->one_minus_albedo * (sink)->one_minus_albedo * (wt) * (wt);         \
+        (sink)->heat[shell] += d_;                                                                   \
+        (sink)->heat2[shell] += d2_;                                                                 \
+    } while (0)
+#include "orc_walk_body.h"
+#undef W_FN
+#undef W_REAL
+#undef W_SRC
+#undef W_SINK
+#undef W_U_STEP
+#undef W_U_DIR
+#undef W_U_ROUL
+#undef W_LOG
+#undef W_SQRT
+#undef W_SINCOS2PI
+#undef W_DEPOSIT
+
+ORC_EXPORT int orc_restate_exact(const orc_params* p, unsigned seed, uint32_t photons, double* heat, double* heat2)
+{
+    if (!orc_params_ok(p) || !heat || !heat2)
+        return -1;
+    long double* acc = calloc(2 * (size_t)p->shells, sizeof(long double));
+    if (!acc)
+        return -2;
+    exact_sink sink = { acc, acc + p->shells, 1.0f - restate_albedo(p) };
+    glibc_src src = { 0 };
+    srand(seed);
+    for (uint32_t i = 0; i < photons; ++i)
+        restate_exact_photon(&src, &sink, restate_albedo(p), restate_spm(p), p->shells, ORC_SAMPLER_REJECTION, 0);
+    for (uint32_t s = 0; s < p->shells; ++s) {
+        heat[s] = (double)acc[s];
+        heat2[s] = (double)acc[p->shells + s];
+    }
+    free(acc);
+    return 0;
+}

@@ -9,7 +9,8 @@ Fixtures (all plain text, small):
   ref_<variant>_s<seed>_n<photons>.txt
       the reference's stdout without its two timing lines
       (tiny_mc.c:55-56), for exact comparison with the oracle's restatement;
-  stats_<variant>_<seeds>x<photons>.csv
+  stats_<variant>_<seeds>x<photons>.csv   (variants *_sm: rand() replaced
+      at link time by SplitMix64, see oracle/ref_rand_splitmix.c)
       per shell: mean and sample standard deviation over seeds 1..<seeds> of
       the reference's heat column (tiny_mc.c:63), and of '# extra'
       (tiny_mc.c:66), for statistical parity of other random streams;
@@ -37,6 +38,11 @@ VARIANTS = {
     "default": ("tiny_mc_default", dict(mu_a=2.0, mu_s=20.0, shells=101, microns_per_shell=50.0)),
     "lowalb": ("tiny_mc_lowalb", dict(mu_a=10.0, mu_s=10.0, shells=51, microns_per_shell=100.0)),
     "highalb": ("tiny_mc_highalb", dict(mu_a=0.5, mu_s=50.0, shells=201, microns_per_shell=20.0)),
+    # the same unmodified sources with rand()/srand() replaced at link time by
+    # SplitMix64 (oracle/ref_rand_splitmix.c): the reference's physics
+    # without glibc rand()'s lag correlations
+    "default_sm": ("tiny_mc_default_sm", dict(mu_a=2.0, mu_s=20.0, shells=101, microns_per_shell=50.0)),
+    "highalb_sm": ("tiny_mc_highalb_sm", dict(mu_a=0.5, mu_s=50.0, shells=201, microns_per_shell=20.0)),
 }
 
 TABLES = [
@@ -48,7 +54,10 @@ TABLES = [
     ("highalb", 11, 2048),
 ]
 
-STATS = [("default", 256, 32768), ("lowalb", 128, 32768), ("highalb", 128, 4096)]
+STATS = [("default", 256, 32768), ("lowalb", 128, 32768), ("highalb", 128, 4096),
+         # many short runs: high precision while the reference's float
+         # accumulation (tiny_mc.c:26-27) stays well below the noise
+         ("default_sm", 8192, 8192), ("highalb_sm", 8192, 1024)]
 
 TIMING = re.compile(r"^# [0-9.]+ (seconds|K photons per second)$")
 
@@ -83,7 +92,8 @@ def main():
         "compiler": subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0],
         "glibc": platform.libc_ver()[1],
         "flags": "-std=c11 -O2 -ffp-contract=off -DSEED='atoi(getenv(\"TMC_SEED\"))' "
-                 "-DPHOTONS='atoi(getenv(\"TMC_PHOTONS\"))' (+ per-variant -D physics)",
+                 "-DPHOTONS='atoi(getenv(\"TMC_PHOTONS\"))' (+ per-variant -D physics; *_sm variants also "
+                 "-Wl,--wrap=rand,--wrap=srand oracle/ref_rand_splitmix.c)",
         "variants": {k: v[1] for k, v in VARIANTS.items()},
         "files": {},
     }

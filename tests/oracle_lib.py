@@ -166,7 +166,9 @@ def tally_heat(tally, shells, mu_a, mu_s):
     computed independently of the product library."""
     t = np.asarray(tally, dtype=np.uint64)
     S = shells
-    albedo = float(mu_s) / (float(mu_s) + float(mu_a))
+    # the parameters are binary32 in the C ABI; the conversion is in double
+    mu_a, mu_s = float(np.float32(mu_a)), float(np.float32(mu_s))
+    albedo = mu_s / (mu_s + mu_a)
     oma = 1.0 - albedo
 
     def u128(lo, hi):

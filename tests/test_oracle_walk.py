@@ -1,0 +1,125 @@
+"""Closed-form checks that pin the FP64 oracle (orc_walk64) and the FP32
+mirror (orc_mirror32) on the Philox stream, where the reference's own
+stream cannot be used: weight cadence, absorption count, energy balance,
+first-step shell law, direction moments, agreement of the two precisions,
+and the accuracy of the mirror's elementary functions."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib
+
+DEFAULT = dict(mu_a=2.0, mu_s=20.0, shells=101, microns_per_shell=50.0)
+# P(u01 <= 0.1f): u01 = (m + 1) 2^-24, m = 0 .. 2^24 - 1 (photon.c:46)
+P_SURVIVE = math.floor(float(np.float32(0.1)) * 2**24) / 2**24
+
+
+def test_weight_cadence(orc):
+    """w_k = a^k until the weight drops below 0.001 (photon.c:32,45): at
+    default physics the first roulette follows absorption 73, later ones
+    every 24 absorptions, so every walk has 73 + 24 j absorptions."""
+    p = orc.params(**DEFAULT)
+    a = 20.0 / 22.0
+    counts = []
+    survivor = None
+    for photon in range(2000):
+        sh, w, n = orc.trace64(p, 7, photon)
+        counts.append(n)
+        assert w[0] == 1.0
+        np.testing.assert_allclose(w[:73], a ** np.arange(73), rtol=1e-12)
+        if n > 97 and survivor is None:
+            survivor = w
+    assert all((n - 73) % 24 == 0 and n >= 73 for n in counts)
+    assert survivor is not None
+    np.testing.assert_allclose(survivor[73], a**73 / float(np.float32(0.1)), rtol=1e-12)
+    assert survivor[72] >= 0.001 and survivor[73] * 0.1 < 0.001
+
+
+@pytest.mark.parametrize("which", ["walk64", "mirror32"])
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_mean_interactions_and_energy(orc, which, sampler):
+    """Mean absorptions 73 + 24 p / (1 - p); absorbed energy per photon -> 1
+    (sum over shells of (1 - a) w, photon.c:30, with an unbiased roulette)."""
+    p = orc.params(**DEFAULT)
+    N = 1 << 16
+    if which == "walk64":
+        heat, _, n = orc.walk64(p, 123, N, sampler=sampler)
+    else:
+        t, n = orc.mirror32(p, 123, N, sampler=sampler)
+        heat, _ = oracle_lib.tally_heat(t, 101, 2.0, 20.0)
+    mean = n / N
+    expect = 73 + 24 * P_SURVIVE / (1 - P_SURVIVE)
+    se = 24 * math.sqrt(P_SURVIVE) / (1 - P_SURVIVE) / math.sqrt(N)
+    assert abs(mean - expect) < 5 * se
+    assert abs(heat.sum() / N - 1.0) < 1e-4
+
+
+def test_first_step_shell_law(orc):
+    """With one absorption per photon the deposit lies at distance t ~ Exp(1)
+    mean free paths along +z (photon.c:12-24), so shell s (of width 1/spm mfp)
+    has probability exp(-s/spm) - exp(-(s+1)/spm); chi-square over shells."""
+    p = orc.params(**DEFAULT)
+    N = 1 << 18
+    heat, heat2, n = orc.walk64(p, 99, N, max_k=1)
+    assert n == N
+    oma = 1.0 - 20.0 / 22.0
+    counts = heat / oma
+    np.testing.assert_allclose(heat2, oma * heat, rtol=1e-12)
+    spm = 1e4 / 50.0 / 22.0
+    s = np.arange(101)
+    prob = np.exp(-s / spm) - np.exp(-(s + 1) / spm)
+    prob[-1] = np.exp(-100 / spm)
+    expect = N * prob
+    keep = expect > 5
+    chi2 = float((((counts - expect) ** 2) / expect)[keep].sum())
+    dof = int(keep.sum()) - 1
+    assert chi2 < dof + 6 * math.sqrt(2 * dof)
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_direction_moments(orc, sampler):
+    """Isotropic scattering (g = 0): unit vectors with E[u] = 0, E[u^2] = 1/3,
+    E[uv] = 0 — the law of the reference's rejection method (photon.c:34-43)."""
+    d = np.concatenate([orc.directions64(5, ph, sampler, 4096) for ph in range(64)])
+    n = d.shape[0]
+    np.testing.assert_allclose(np.einsum("ij,ij->i", d, d), 1.0, atol=1e-12)
+    se1 = math.sqrt(1 / 3 / n)
+    se2 = math.sqrt((1 / 5 - 1 / 9) / n)
+    assert np.all(np.abs(d.mean(axis=0)) < 5 * se1)
+    assert np.all(np.abs((d * d).mean(axis=0) - 1 / 3) < 5 * se2)
+    cross = np.array([(d[:, 0] * d[:, 1]).mean(), (d[:, 0] * d[:, 2]).mean(), (d[:, 1] * d[:, 2]).mean()])
+    assert np.all(np.abs(cross) < 5 * math.sqrt(1 / 15 / n))
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+@pytest.mark.parametrize("phys", [DEFAULT, dict(mu_a=10.0, mu_s=10.0, shells=51, microns_per_shell=100.0)])
+def test_mirror32_agrees_with_walk64(orc, sampler, phys):
+    """The FP32 mirror and the FP64 oracle walk the same uniforms: the same
+    roulette decisions (identical absorption counts) and per-shell heat equal
+    to rounding level, except where a deposit lands within rounding of a shell
+    boundary."""
+    p = orc.params(**phys)
+    N = 1 << 15
+    h64, _, n64 = orc.walk64(p, 31337, N, sampler=sampler)
+    t, n32 = orc.mirror32(p, 31337, N, sampler=sampler)
+    h32, _ = oracle_lib.tally_heat(t, phys["shells"], phys["mu_a"], phys["mu_s"])
+    assert n32 == n64
+    assert abs(h32.sum() / h64.sum() - 1) < 1e-6
+    big = h64 > 1e-3 * h64.max()
+    assert np.all(np.abs(h32[big] / h64[big] - 1) < 2e-3)
+
+
+def test_mirror_math_accuracy(orc):
+    """Own log and sin/cos(2 pi a) over every uniform the walk draws."""
+    log_ulp, sc_abs, sc_ulp = orc.math32_errors()
+    assert log_ulp <= 1.0
+    assert sc_abs <= 1.2e-7
+    assert sc_ulp <= 2.0
+
+
+def test_derived_constants_match_reference_arithmetic(orc):
+    """albedo in float and shells_per_mfp via a double quotient, photon.c:8-9."""
+    a, spm = orc.derived32(orc.params(**DEFAULT))
+    assert a == np.float32(np.float32(20.0) / np.float32(22.0))
+    assert spm == np.float32(1e4 / 50.0 / float(np.float32(22.0)))

@@ -35,7 +35,8 @@ def test_driver_table_matches_mirror(tmc, orc):
     heat, heat2 = oracle_lib.tally_heat(tally, 101, 2.0, 20.0)
     rows_want, extra = tmc.table_rows(heat, heat2, 65536, 50.0)
     want = np.array(rows_want)
-    np.testing.assert_allclose(got, want, rtol=0, atol=1.5e-5 + 1e-9 * np.abs(want))
+    # printed with 5 decimals (%12.5f)
+    assert np.all(np.abs(got - want) <= 1.5e-5 + 1e-9 * np.abs(want))
     assert abs(float(lines[-1].split("\t")[1]) - extra) <= 1.5e-5
     for i, r in enumerate(rows):
         assert r[0] == "%6.0f" % (i * 50.0)

@@ -124,6 +124,77 @@ const char* tiny_mc_gpu_strerror(int status);
 int tiny_mc_gpu_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out);
 int tiny_mc_gpu_eval_math(const float* in, uint32_t n, float* log_out, float* sin_out, float* cos_out);
 
+/* ------------------------------------------------------------------------
+ * Batched sweep: many parameter sets in one persistent launch.
+ *
+ * No counterpart in the reference (it has one compile-time parameter set);
+ * this is the loop of tiny_mc.c:47-49 run for many params.h configurations
+ * at once.  Each set's tallies are bit-identical to tiny_mc_gpu_run_ex on
+ * that set alone.
+ * ------------------------------------------------------------------------ */
+
+/* One simulation of a batch: physics and photon count, its seed, and the
+ * stream index of its first photon. */
+typedef struct tmc_set {
+    tmc_params params;
+    uint64_t seed;
+    uint64_t first_photon;
+} tmc_set;
+
+typedef struct tmc_batch_options {
+    uint32_t sampler; /* TMC_SAMPLER_* for every set */
+    uint32_t block; /* threads per workgroup, multiple of 64; 0 = auto (256) */
+    uint32_t waves_per_simd; /* resident waves per SIMD, 1..8; 0 = as many as fit */
+    uint32_t copies; /* LDS tally copies per wave: 1 or 2; 0 = auto (1) */
+    uint64_t chunk_work; /* target absorptions per queue chunk; 0 = auto */
+    uint64_t max_interactions; /* per photon, 0 = never stop early */
+    uint32_t use_graph; /* nonzero: plans replay a captured HIP graph */
+    uint32_t reserved;
+} tmc_batch_options;
+
+typedef struct tmc_batch_stats {
+    uint64_t sets;
+    uint64_t photons;
+    uint64_t interactions;
+    uint64_t truncated;
+    uint64_t chunks; /* queue length */
+    uint64_t chunk_work; /* target absorptions per chunk used */
+    double expected_interactions; /* host estimate used for scheduling */
+    double kernel_ms; /* clear + walk on the device (HIP events) */
+    double total_ms; /* whole call */
+    uint32_t grid, block, copies, waves_per_simd;
+} tmc_batch_stats;
+
+typedef struct tmc_batch_plan tmc_batch_plan;
+
+void tmc_batch_options_default(tmc_batch_options* opt);
+
+/* Words of the concatenated tallies of a batch: 4 * sum of shells.  Set i's
+ * tally (layout as tiny_mc_gpu_run_ex) starts at word 4 * sum_{j<i} shells_j. */
+uint64_t tmc_batch_tally_words(const tmc_set* sets, uint32_t nsets);
+
+/* Validates the sets, builds the chunk queue (longest first) and uploads the
+ * set table and queue to the device; with use_graph, captures the steady
+ * launch sequence (clear, walk, copy back) as a HIP graph. */
+int tmc_batch_plan_create(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* opt,
+                          tmc_batch_plan** plan);
+
+/* Runs the plan; tallies receives tmc_batch_tally_words() words. */
+int tmc_batch_plan_run(tmc_batch_plan* plan, uint64_t* tallies, tmc_batch_stats* stats);
+
+void tmc_batch_plan_destroy(tmc_batch_plan* plan);
+
+/* create + run + destroy. */
+int tmc_batch_run(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* opt, uint64_t* tallies,
+                  tmc_batch_stats* stats);
+
+/* Picks block size, resident waves, tally copies and chunk work for this
+ * batch by timing candidate launches on a reduced copy of it (every set's
+ * photon budget divided by `sample_divisor`, at least 64 photons).  Writes
+ * the fastest options to *best (starting from *base). */
+int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* base,
+                       uint32_t sample_divisor, tmc_batch_options* best);
+
 #ifdef __cplusplus
 }
 #endif

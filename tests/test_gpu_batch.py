@@ -1,0 +1,107 @@
+"""Batched sweep (many parameter sets in one persistent launch): every set's
+tallies equal the same set run alone (bit for bit, tiny_mc_gpu_run_ex) and
+the oracle's FP32 mirror, under every launch shape, chunking, tally
+privatisation and in HIP-graph replays."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def small_sweep(tmc, n=48, scale=1.0 / 256, seed=5):
+    return tmc.sweep_sets(n=n, seed=seed, photon_scale=scale)
+
+
+def alone(tmc, sets, sampler=0):
+    out = []
+    for p, seed, *rest in sets:
+        t, _ = tmc.run_raw(p, seed, sampler=sampler, first_photon=rest[0] if rest else 0)
+        out.append(t)
+    return out
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_batch_equals_sets_alone(tmc, sampler):
+    sets = small_sweep(tmc)
+    flat, st = tmc.batch_run(sets, tmc.BatchOptions(sampler=sampler))
+    parts = tmc.split_tallies(sets, flat)
+    ref = alone(tmc, sets, sampler)
+    for i, (a, b) in enumerate(zip(parts, ref)):
+        np.testing.assert_array_equal(a, b, err_msg=f"set {i}")
+    assert st.sets == len(sets) and st.photons == sum(s[0].photons for s in sets)
+
+
+def test_batch_equals_mirror(tmc, orc):
+    sets = small_sweep(tmc, n=6, scale=1.0 / 1024, seed=9)
+    flat, st = tmc.batch_run(sets)
+    total = 0
+    for (p, seed), t in zip(sets, tmc.split_tallies(sets, flat)):
+        want, n = orc.mirror32(orc.params(p.mu_a, p.mu_s, p.shells, p.microns_per_shell), seed, p.photons)
+        np.testing.assert_array_equal(t, want)
+        total += n
+    assert st.interactions == total
+
+
+@pytest.mark.parametrize("opts", [
+    dict(block=64), dict(block=1024, copies=1), dict(copies=2), dict(waves_per_simd=2),
+    dict(chunk_work=4096), dict(chunk_work=1 << 30), dict(block=512, copies=2, chunk_work=100000),
+])
+def test_launch_shapes_identical(tmc, opts):
+    sets = small_sweep(tmc, n=24, seed=11)
+    base, _ = tmc.batch_run(sets)
+    got, st = tmc.batch_run(sets, tmc.BatchOptions(**opts))
+    np.testing.assert_array_equal(got, base)
+
+
+def test_first_photon_offsets_and_empty_sets(tmc):
+    p = tmc.Params(photons=5000)
+    sets = [(p, 3, 0), (tmc.Params(photons=0), 4), (p, 3, 5000), (tmc.Params(photons=1, shells=1), 8),
+            (p, 3, (1 << 32) - 2500)]
+    flat, st = tmc.batch_run(sets)
+    parts = tmc.split_tallies(sets, flat)
+    np.testing.assert_array_equal(parts[0], tmc.run_raw(p, 3)[0])
+    assert not parts[1].any()
+    np.testing.assert_array_equal(parts[2], tmc.run_raw(p, 3, first_photon=5000)[0])
+    np.testing.assert_array_equal(parts[3], tmc.run_raw(tmc.Params(photons=1, shells=1), 8)[0])
+    np.testing.assert_array_equal(parts[4], tmc.run_raw(p, 3, first_photon=(1 << 32) - 2500)[0])
+    flat0, st0 = tmc.batch_run([])
+    assert flat0.size == 0 and st0.photons == 0
+
+
+def test_graph_replays_identical(tmc):
+    """A captured plan replays to the same bits every time, with eager work
+    (other launches and synchronisations) between the replays."""
+    sets = small_sweep(tmc, n=32, seed=21)
+    eager, _ = tmc.batch_run(sets)
+    with tmc.BatchPlan(sets, tmc.BatchOptions(use_graph=True)) as plan:
+        for r in range(4):
+            got, st = plan.run()
+            np.testing.assert_array_equal(got, eager, err_msg=f"replay {r}")
+            tmc.run_raw(tmc.Params(photons=10000), r)  # eager work in between
+
+
+def test_autotune_returns_working_options(tmc):
+    sets = small_sweep(tmc, n=32, seed=33)
+    best = tmc.batch_autotune(sets, sample_divisor=4)
+    assert best.chunk_work > 0
+    got, st = tmc.batch_run(sets, best)
+    base, _ = tmc.batch_run(sets)
+    np.testing.assert_array_equal(got, base)
+
+
+def test_truncation_in_batch(tmc):
+    sets = small_sweep(tmc, n=8, seed=2)
+    flat, st = tmc.batch_run(sets, tmc.BatchOptions(max_interactions=10))
+    for (p, seed), t in zip(sets, tmc.split_tallies(sets, flat)):
+        np.testing.assert_array_equal(t, tmc.run_raw(p, seed, max_interactions=10)[0])
+    assert st.truncated > 0
+
+
+def test_max_shells(tmc):
+    """LDS slot of 8192 shells (one wave per workgroup) and the single-run
+    kernel at the same size."""
+    p = tmc.Params(shells=8192, microns_per_shell=1.0, photons=20000)
+    flat, st = tmc.batch_run([(p, 1)])
+    assert st.block == 64
+    single, _ = tmc.run_raw(p, 1)
+    np.testing.assert_array_equal(flat, single)

@@ -7,6 +7,12 @@ params.h macros become ``Params`` fields and the photon() loop of
 tiny_mc.c:43-49 becomes one ``run`` call.
 """
 from ._lib import (  # noqa: F401
+    BatchOptions,
+    BatchPlan,
+    BatchStats,
+    batch_autotune,
+    batch_run,
+    split_tallies,
     HEAT2_FRAC_BITS,
     HEAT_FRAC_BITS,
     MAX_SHELLS,
@@ -25,4 +31,5 @@ from ._lib import (  # noqa: F401
     shutdown,
     tally_to_heat,
 )
+from .sweep import sweep_sets  # noqa: F401
 from .table import format_table, table_rows  # noqa: F401

@@ -219,3 +219,192 @@ def device_info():
 def shutdown():
     if _lib is not None:
         _lib.tiny_mc_gpu_shutdown()
+
+
+# ---------------------------------------------------------------------------
+# Batched sweep (tmc_batch_*)
+# ---------------------------------------------------------------------------
+
+
+class _Set(ctypes.Structure):
+    _fields_ = [("params", _Params), ("seed", ctypes.c_uint64), ("first_photon", ctypes.c_uint64)]
+
+
+class _BatchOptions(ctypes.Structure):
+    _fields_ = [
+        ("sampler", ctypes.c_uint32),
+        ("block", ctypes.c_uint32),
+        ("waves_per_simd", ctypes.c_uint32),
+        ("copies", ctypes.c_uint32),
+        ("chunk_work", ctypes.c_uint64),
+        ("max_interactions", ctypes.c_uint64),
+        ("use_graph", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class _BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("sets", ctypes.c_uint64),
+        ("photons", ctypes.c_uint64),
+        ("interactions", ctypes.c_uint64),
+        ("truncated", ctypes.c_uint64),
+        ("chunks", ctypes.c_uint64),
+        ("chunk_work", ctypes.c_uint64),
+        ("expected_interactions", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("grid", ctypes.c_uint32),
+        ("block", ctypes.c_uint32),
+        ("copies", ctypes.c_uint32),
+        ("waves_per_simd", ctypes.c_uint32),
+    ]
+
+
+@dataclass
+class BatchStats:
+    sets: int
+    photons: int
+    interactions: int
+    truncated: int
+    chunks: int
+    chunk_work: int
+    expected_interactions: float
+    kernel_ms: float
+    total_ms: float
+    grid: int
+    block: int
+    copies: int
+    waves_per_simd: int
+
+
+@dataclass
+class BatchOptions:
+    sampler: int = SAMPLER_ANALYTIC
+    block: int = 0
+    waves_per_simd: int = 0
+    copies: int = 0
+    chunk_work: int = 0
+    max_interactions: int = 0
+    use_graph: bool = False
+
+    def _c(self):
+        return _BatchOptions(self.sampler, self.block, self.waves_per_simd, self.copies, self.chunk_work,
+                             self.max_interactions, int(self.use_graph), 0)
+
+
+def _bind_batch(lib):
+    if getattr(lib, "_tmc_batch_bound", False):
+        return
+    P = ctypes.POINTER
+    lib.tmc_batch_tally_words.argtypes = [P(_Set), ctypes.c_uint32]
+    lib.tmc_batch_tally_words.restype = ctypes.c_uint64
+    lib.tmc_batch_plan_create.argtypes = [P(_Set), ctypes.c_uint32, P(_BatchOptions), P(ctypes.c_void_p)]
+    lib.tmc_batch_plan_create.restype = ctypes.c_int
+    lib.tmc_batch_plan_run.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(_BatchStats)]
+    lib.tmc_batch_plan_run.restype = ctypes.c_int
+    lib.tmc_batch_plan_destroy.argtypes = [ctypes.c_void_p]
+    lib.tmc_batch_plan_destroy.restype = None
+    lib.tmc_batch_run.argtypes = [P(_Set), ctypes.c_uint32, P(_BatchOptions), P(ctypes.c_uint64), P(_BatchStats)]
+    lib.tmc_batch_run.restype = ctypes.c_int
+    lib.tmc_batch_autotune.argtypes = [P(_Set), ctypes.c_uint32, P(_BatchOptions), ctypes.c_uint32,
+                                       P(_BatchOptions)]
+    lib.tmc_batch_autotune.restype = ctypes.c_int
+    lib.tmc_batch_options_default.argtypes = [P(_BatchOptions)]
+    lib.tmc_batch_options_default.restype = None
+    lib._tmc_batch_bound = True
+
+
+def _sets_array(sets):
+    """sets: iterable of (Params, seed) or (Params, seed, first_photon)."""
+    sets = list(sets)
+    arr = (_Set * max(1, len(sets)))()
+    for i, s in enumerate(sets):
+        p, seed = s[0], s[1]
+        first = s[2] if len(s) > 2 else 0
+        arr[i] = _Set(p._c(), seed, first)
+    return arr, len(sets)
+
+
+def _batch_stats(st):
+    return BatchStats(st.sets, st.photons, st.interactions, st.truncated, st.chunks, st.chunk_work,
+                      st.expected_interactions, st.kernel_ms, st.total_ms, st.grid, st.block, st.copies,
+                      st.waves_per_simd)
+
+
+def split_tallies(sets, flat):
+    """Per-set views (4*shells words each) of concatenated batch tallies."""
+    out, off = [], 0
+    for s in sets:
+        n = 4 * s[0].shells
+        out.append(flat[off:off + n])
+        off += n
+    return out
+
+
+class BatchPlan:
+    """A batch uploaded once and run many times (optionally from a HIP graph)."""
+
+    def __init__(self, sets, options: BatchOptions = None):
+        lib = load()
+        _bind_batch(lib)
+        self._lib = lib
+        self.sets = list(sets)
+        self._arr, self._n = _sets_array(self.sets)
+        self.tally_words = lib.tmc_batch_tally_words(self._arr, self._n)
+        opt = (options or BatchOptions())._c()
+        h = ctypes.c_void_p()
+        _check(lib.tmc_batch_plan_create(self._arr, self._n, ctypes.byref(opt), ctypes.byref(h)),
+               "tmc_batch_plan_create")
+        self._h = h
+
+    def run(self):
+        t = np.zeros(max(1, self.tally_words), dtype=np.uint64)
+        st = _BatchStats()
+        _check(self._lib.tmc_batch_plan_run(self._h, _ptr(t, ctypes.c_uint64), ctypes.byref(st)),
+               "tmc_batch_plan_run")
+        return t[:self.tally_words], _batch_stats(st)
+
+    def close(self):
+        if self._h:
+            self._lib.tmc_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_run(sets, options: BatchOptions = None):
+    """Runs a batch once: (concatenated tallies, BatchStats)."""
+    lib = load()
+    _bind_batch(lib)
+    sets = list(sets)
+    arr, n = _sets_array(sets)
+    words = lib.tmc_batch_tally_words(arr, n)
+    t = np.zeros(max(1, words), dtype=np.uint64)
+    st = _BatchStats()
+    opt = (options or BatchOptions())._c()
+    _check(lib.tmc_batch_run(arr, n, ctypes.byref(opt), _ptr(t, ctypes.c_uint64), ctypes.byref(st)),
+           "tmc_batch_run")
+    return t[:words], _batch_stats(st)
+
+
+def batch_autotune(sets, base: BatchOptions = None, sample_divisor=16):
+    lib = load()
+    _bind_batch(lib)
+    arr, n = _sets_array(sets)
+    b = (base or BatchOptions())._c()
+    out = _BatchOptions()
+    _check(lib.tmc_batch_autotune(arr, n, ctypes.byref(b), sample_divisor, ctypes.byref(out)),
+           "tmc_batch_autotune")
+    return BatchOptions(out.sampler, out.block, out.waves_per_simd, out.copies, out.chunk_work,
+                        out.max_interactions, bool(out.use_graph))

@@ -15,22 +15,14 @@
 #include <mutex>
 
 #include "../../include/tiny_mc_gpu.h"
+#include "tmc_context.h"
 #include "tmc_internal.h"
 
 #define TMC_EXPORT extern "C" __attribute__((visibility("default")))
 
-namespace {
+using tmc::Context;
 
-struct Context {
-    int device = -1;
-    int compute_units = 0;
-    char arch[64] = { 0 };
-    hipStream_t stream = nullptr;
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-    unsigned long long* d_tally = nullptr;
-    size_t tally_words = 0;
-    unsigned long long* d_counters = nullptr; /* 4 words */
-};
+namespace {
 
 std::mutex g_mutex;
 Context* g_ctx = nullptr;
@@ -90,7 +82,11 @@ int create_context(int device, Context** out)
     return TMC_OK;
 }
 
-int get_context(Context** out)
+} // namespace
+
+std::mutex& tmc::api_mutex() { return g_mutex; }
+
+int tmc::acquire_context(Context** out)
 {
     if (!g_ctx) {
         int rc = create_context(-1, &g_ctx);
@@ -102,6 +98,8 @@ int get_context(Context** out)
     *out = g_ctx;
     return TMC_OK;
 }
+
+namespace {
 
 int ensure_tally(Context* c, size_t words)
 {
@@ -117,10 +115,12 @@ int ensure_tally(Context* c, size_t words)
     return TMC_OK;
 }
 
+} // namespace
+
 /* Parameter checks.  The reference has none (params.h:14 only warns that
  * MU_A must be non-zero: with mu_a = 0 the weight never reaches the cutoff
  * and photon() never returns). */
-int check_params(const tmc_params* p)
+int tmc::check_params(const tmc_params* p)
 {
     if (!p)
         return TMC_E_INVALID;
@@ -142,7 +142,7 @@ int check_params(const tmc_params* p)
 }
 
 /* photon.c:8: albedo evaluated in float. */
-float derived_albedo(const tmc_params* p)
+float tmc::derived_albedo(const tmc_params* p)
 {
     const volatile float sum = p->mu_s + p->mu_a;
     return p->mu_s / sum;
@@ -150,11 +150,51 @@ float derived_albedo(const tmc_params* p)
 
 /* photon.c:9: 1e4 is a double literal, so the quotient is formed in double
  * from the float sum mu_a + mu_s, then stored as float. */
-float derived_spm(const tmc_params* p)
+float tmc::derived_spm(const tmc_params* p)
 {
     const volatile float sum = p->mu_a + p->mu_s;
     return (float)(1e4 / (double)p->microns_per_shell / (double)sum);
 }
+
+/* The weight after k absorptions is a deterministic binary32 sequence
+ * (photon.c:32, 45-49): it falls below 0.001 after k1 steps, then every k2
+ * steps after each survival, so the mean is k1 + k2 p / (1 - p) with p the
+ * survival probability 1677721 / 2^24 of the (0,1] uniform. */
+double tmc::expected_interactions(const tmc_params* p, uint64_t kcap)
+{
+    const float a = derived_albedo(p);
+    const double pr = 1677721.0 / 16777216.0;
+    const uint64_t limit = 1u << 20;
+    /* first cycle: multiply from 1 until below the cutoff */
+    float w = 1.0f;
+    uint64_t k1 = 0;
+    while (w >= 0.001f && k1 < limit) {
+        w = w * a;
+        ++k1;
+    }
+    double c1, c2;
+    if (w < 0.001f) {
+        c1 = (double)k1;
+        /* later cycles start from the cutoff weight / 0.1 */
+        float v = w / 0.1f;
+        uint64_t k2 = 0;
+        while (v >= 0.001f && k2 < limit) {
+            v = v * a;
+            ++k2;
+        }
+        c2 = (double)(k2 ? k2 : 1);
+    } else {
+        /* albedo so close to 1 that the cadence is long: continuous estimate */
+        c1 = std::log(0.001) / std::log((double)a);
+        c2 = std::log(0.1) / std::log((double)a);
+    }
+    double mean = c1 + c2 * pr / (1.0 - pr);
+    if (kcap && mean > (double)kcap)
+        mean = (double)kcap;
+    return mean < 1.0 ? 1.0 : mean;
+}
+
+namespace {
 
 double u128_to_double(uint64_t lo, uint64_t hi)
 {
@@ -202,8 +242,8 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     a.seed = seed;
     a.first_photon = o->first_photon;
     a.photons = p->photons;
-    a.albedo = derived_albedo(p);
-    a.spm = derived_spm(p);
+    a.albedo = tmc::derived_albedo(p);
+    a.spm = tmc::derived_spm(p);
     a.shell_max = (float)(S - 1);
     a.shells = S;
     a.chunk = chunk;
@@ -267,7 +307,7 @@ TMC_EXPORT void tmc_options_default(tmc_options* opt)
 TMC_EXPORT int tiny_mc_gpu_run_ex(const tmc_params* p, uint64_t seed, const tmc_options* opt, uint64_t* tally,
                                   tmc_run_stats* stats)
 {
-    int rc = check_params(p);
+    int rc = tmc::check_params(p);
     if (rc != TMC_OK)
         return rc;
     if (!tally)
@@ -279,7 +319,7 @@ TMC_EXPORT int tiny_mc_gpu_run_ex(const tmc_params* p, uint64_t seed, const tmc_
         tmc_options_default(&o);
     std::lock_guard<std::mutex> lock(g_mutex);
     Context* c = nullptr;
-    rc = get_context(&c);
+    rc = tmc::acquire_context(&c);
     if (rc != TMC_OK)
         return rc;
     return run_locked(c, p, seed, &o, tally, stats);
@@ -287,7 +327,7 @@ TMC_EXPORT int tiny_mc_gpu_run_ex(const tmc_params* p, uint64_t seed, const tmc_
 
 TMC_EXPORT int tiny_mc_gpu_tally_to_heat(const tmc_params* p, const uint64_t* tally, double* heat, double* heat2)
 {
-    int rc = check_params(p);
+    int rc = tmc::check_params(p);
     if (rc != TMC_OK)
         return rc;
     if (!tally || !heat || !heat2)
@@ -308,7 +348,7 @@ TMC_EXPORT int tiny_mc_gpu_tally_to_heat(const tmc_params* p, const uint64_t* ta
 
 TMC_EXPORT int tiny_mc_gpu_run(const tmc_params* p, uint64_t seed, double* heat, double* heat2)
 {
-    int rc = check_params(p);
+    int rc = tmc::check_params(p);
     if (rc != TMC_OK)
         return rc;
     if (!heat || !heat2)
@@ -345,7 +385,7 @@ TMC_EXPORT int tiny_mc_gpu_device_info(char* name, size_t cap, int* compute_unit
 {
     std::lock_guard<std::mutex> lock(g_mutex);
     Context* c = nullptr;
-    int rc = get_context(&c);
+    int rc = tmc::acquire_context(&c);
     if (rc != TMC_OK)
         return rc;
     if (name && cap)
@@ -390,7 +430,7 @@ TMC_EXPORT int tiny_mc_gpu_philox(const uint32_t* ctr, const uint32_t* key, uint
         return TMC_E_INVALID;
     std::lock_guard<std::mutex> lock(g_mutex);
     Context* c = nullptr;
-    int rc = get_context(&c);
+    int rc = tmc::acquire_context(&c);
     if (rc != TMC_OK || n == 0)
         return rc;
     DevBufs b;
@@ -413,7 +453,7 @@ TMC_EXPORT int tiny_mc_gpu_eval_math(const float* in, uint32_t n, float* log_out
         return TMC_E_INVALID;
     std::lock_guard<std::mutex> lock(g_mutex);
     Context* c = nullptr;
-    int rc = get_context(&c);
+    int rc = tmc::acquire_context(&c);
     if (rc != TMC_OK || n == 0)
         return rc;
     DevBufs b;

@@ -1,0 +1,235 @@
+/* Many parameter sets in one persistent launch (batched sweep).
+ *
+ * The host splits every set (mu_a, mu_s, shells, width, photon budget, seed)
+ * into chunks of photons with about equal expected work, sorts the chunks
+ * longest first, and uploads the set table and the chunk queue once.  Each
+ * wave claims the next chunk from a global cursor and walks its photons with
+ * per-lane regeneration; the set is uniform across the wave, so the set's
+ * physics and Philox key stay in scalar registers.  A wave moves to a chunk
+ * of another set only when all its lanes are dead (the drain), which the
+ * chunk sizes keep short relative to the chunk.
+ *
+ * Tallies: every wave owns an LDS slot of shells_max shells (1 or 2 copies,
+ * lanes 0-31 / 32-63 in separate copies to halve same-address collisions),
+ * flushed into the set's 128-bit global sums when the wave changes set,
+ * every flush_interval steps, and at exit.  After each flush the wave adds
+ * the photons it completed for that set to the set's done counter (release
+ * ordering), so done == photons means the set's tallies are complete.
+ *
+ * The per-absorption arithmetic is the same step<>() as the single-run
+ * kernel, so each set's tallies are bit-identical to a single run of that
+ * set (the stream is keyed by (seed, photon index) only).
+ */
+#include "tmc_batch_internal.h"
+#include "tmc_device.h"
+#include "tmc_step.h"
+
+namespace tmc {
+
+/* Moves one wave's private LDS slot into a set's global sums and zeroes it.
+ * Only this wave touches the slot, and a wave's LDS operations complete in
+ * order, so plain loads see every earlier ds_add of the wave. */
+__device__ __forceinline__ void flush_slot(unsigned long long* slot, uint32_t shells_max, uint32_t copies,
+                                           uint32_t shells, unsigned long long* g, uint32_t lane)
+{
+    for (uint32_t s = lane; s < shells; s += 64) {
+        unsigned long long a = 0, b = 0;
+        for (uint32_t c = 0; c < copies; ++c) {
+            unsigned long long* lw = slot + (size_t)c * 2 * shells_max;
+            a += lw[s];
+            b += lw[shells_max + s];
+            lw[s] = 0;
+            lw[shells_max + s] = 0;
+        }
+        if (a)
+            add128(g + 2 * s, a);
+        if (b)
+            add128(g + 2 * shells + 2 * s, b);
+    }
+}
+
+/* Uniform view of the set a wave is walking. */
+struct SetState {
+    Physics phys;
+    uint32_t shells;
+    uint32_t kcap;
+    uint64_t first_photon;
+    unsigned long long* tally;
+    unsigned long long* done;
+};
+
+__device__ __forceinline__ void load_set(SetState& st, const BatchSet* sets, uint32_t set,
+                                         unsigned long long* pool, unsigned long long* done)
+{
+    const BatchSet& d = sets[set];
+    st.phys.albedo = d.albedo;
+    st.phys.spm = d.spm;
+    st.phys.shell_max = d.shell_max;
+    st.phys.last_shell = d.shells - 1;
+    st.phys.key0 = (uint32_t)d.seed;
+    st.phys.key1 = (uint32_t)(d.seed >> 32);
+    st.shells = d.shells;
+    st.kcap = d.kcap;
+    st.first_photon = d.first_photon;
+    st.tally = pool + d.tally_offset;
+    st.done = done + set;
+}
+
+template <uint32_t SAMPLER>
+__global__ void __launch_bounds__(1024) walk_batch(BatchArgs a)
+{
+    extern __shared__ unsigned long long lds_slots[];
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    unsigned long long* slot = lds_slots + (size_t)wave * a.copies * 2 * a.shells_max;
+    for (uint32_t i = lane; i < a.copies * 2 * a.shells_max; i += 64)
+        slot[i] = 0;
+    unsigned long long* lw = slot + (a.copies == 2 && lane >= 32 ? 2 * a.shells_max : 0);
+    unsigned long long* lw2 = lw + a.shells_max;
+
+    SetState st;
+    uint32_t cur_set = 0xFFFFFFFFu;
+    uint64_t pool_next = 0, pool_end = 0; /* photon offsets within cur_set */
+    bool have_pending = false, queue_done = false;
+    BatchChunk pend = { 0u, 0u, 0ull };
+    uint64_t wave_interactions = 0, wave_truncated = 0, completed = 0;
+    uint32_t since_flush = 0;
+
+    bool alive = false;
+    Photon ph;
+    launch_photon(ph, 0);
+
+    for (;;) {
+        uint64_t dead = __ballot(!alive);
+        if (dead != 0) {
+            if (pool_next == pool_end && !have_pending && !queue_done) {
+                unsigned long long c = 0;
+                if (lane == 0)
+                    c = atomicAdd(a.counters, 1ull);
+                c = rfl64(c);
+                if (c >= a.nchunks) {
+                    queue_done = true;
+                } else {
+                    const BatchChunk ch = a.chunks[c];
+                    if (ch.set == cur_set) {
+                        pool_next = ch.first;
+                        pool_end = ch.first + ch.count;
+                    } else {
+                        pend = ch;
+                        have_pending = true;
+                    }
+                }
+            }
+            if (have_pending && dead == ~0ull) {
+                /* drained: hand the finished set's sums over, switch sets */
+                if (cur_set != 0xFFFFFFFFu) {
+                    flush_slot(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
+                    __threadfence();
+                    if (lane == 0 && completed)
+                        atomicAdd(st.done, (unsigned long long)completed);
+                    completed = 0;
+                    since_flush = 0;
+                }
+                cur_set = pend.set;
+                load_set(st, a.sets, cur_set, a.tally, a.done);
+                pool_next = pend.first;
+                pool_end = pend.first + pend.count;
+                have_pending = false;
+            }
+            const uint64_t avail = pool_end - pool_next;
+            if (avail) {
+                const uint32_t ndead = (uint32_t)__popcll(dead);
+                const uint32_t rank = mask_rank(dead);
+                if (!alive && rank < avail) {
+                    launch_photon(ph, st.first_photon + pool_next + rank);
+                    alive = true;
+                }
+                pool_next += avail < ndead ? avail : ndead;
+            }
+        }
+        const uint64_t live = __ballot(alive);
+        if (live == 0) {
+            if (queue_done && !have_pending)
+                break;
+            continue;
+        }
+        wave_interactions += (uint64_t)__popcll(live);
+
+        bool truncated = false;
+        if (alive) {
+            alive = step<SAMPLER>(ph, st.phys, lw, lw2);
+            if (alive && ph.k >= st.kcap) {
+                alive = false;
+                truncated = true;
+            }
+        }
+        wave_truncated += (uint64_t)__popcll(__ballot(truncated));
+        completed += (uint64_t)__popcll(live & __ballot(!alive));
+
+        if (++since_flush >= a.flush_interval) {
+            since_flush = 0;
+            flush_slot(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
+            __threadfence();
+            if (lane == 0 && completed)
+                atomicAdd(st.done, (unsigned long long)completed);
+            completed = 0;
+        }
+    }
+    if (cur_set != 0xFFFFFFFFu) {
+        flush_slot(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
+        __threadfence();
+        if (lane == 0 && completed)
+            atomicAdd(st.done, (unsigned long long)completed);
+    }
+    if (lane == 0) {
+        atomicAdd(a.counters + 1, (unsigned long long)wave_interactions);
+        if (wave_truncated)
+            atomicAdd(a.counters + 2, (unsigned long long)wave_truncated);
+    }
+}
+
+/* Clears the tally pool, the done counters and the queue counters (a
+ * kernel rather than a memset so the sequence can be captured in a graph). */
+__global__ void batch_clear(unsigned long long* tally, uint64_t tally_words, unsigned long long* done,
+                            uint32_t nsets, unsigned long long* counters)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tally_words; i += stride)
+        tally[i] = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nsets; i += stride)
+        done[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 4)
+        counters[threadIdx.x] = 0;
+}
+
+size_t batch_lds_bytes(uint32_t block, uint32_t shells_max, uint32_t copies)
+{
+    return (size_t)(block / 64) * copies * 2 * shells_max * sizeof(unsigned long long);
+}
+
+hipError_t launch_batch(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream)
+{
+    const size_t lds = batch_lds_bytes(block, a.shells_max, a.copies);
+    if (sampler == 1u)
+        hipLaunchKernelGGL(walk_batch<1u>, dim3(grid), dim3(block), lds, stream, a);
+    else
+        hipLaunchKernelGGL(walk_batch<0u>, dim3(grid), dim3(block), lds, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_clear(unsigned long long* tally, uint64_t tally_words, unsigned long long* done,
+                              uint32_t nsets, unsigned long long* counters, hipStream_t stream)
+{
+    hipLaunchKernelGGL(batch_clear, dim3(1024), dim3(256), 0, stream, tally, tally_words, done, nsets, counters);
+    return hipGetLastError();
+}
+
+hipError_t batch_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max, uint32_t copies, int* wg_per_cu)
+{
+    const size_t lds = batch_lds_bytes(block, shells_max, copies);
+    if (sampler == 1u)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_batch<1u>, (int)block, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_batch<0u>, (int)block, lds);
+}
+
+} // namespace tmc

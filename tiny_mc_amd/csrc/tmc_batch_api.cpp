@@ -1,0 +1,489 @@
+/* Host side of the batched sweep (include/tiny_mc_gpu.h, tmc_batch_*).
+ *
+ * A plan owns the device copies of the set table and the chunk queue, the
+ * tally pool and, optionally, a HIP graph of the steady launch sequence
+ * (clear kernel, persistent walk kernel, copy-back), so repeated runs of the
+ * same batch cost one graph launch.
+ *
+ * Scheduling: each set's expected absorptions per photon L follow from the
+ * binary32 weight cadence (tmc::expected_interactions).  A set of N photons
+ * is cut into chunks of 64 * P photons with P = ceil(chunk_work / (64 L)),
+ * so every chunk carries about chunk_work absorptions and a wave drains (its
+ * lanes run out of the set's photons) once per ~P photons per lane.  Chunks
+ * are sorted by expected duration, longest first, which keeps the long
+ * high-albedo chunks off the end of the launch.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/tiny_mc_gpu.h"
+#include "tmc_batch_internal.h"
+#include "tmc_context.h"
+
+#define TMC_EXPORT extern "C" __attribute__((visibility("default")))
+
+struct tmc_batch_plan {
+    uint32_t nsets = 0;
+    uint64_t tally_words = 0;
+    uint64_t nchunks = 0;
+    uint64_t photons = 0;
+    double expected = 0.0;
+    uint64_t chunk_work = 0;
+    uint32_t sampler = 0, grid = 0, block = 0, copies = 1, wps = 0, shells_max = 1;
+    bool use_graph = false;
+    tmc::BatchSet* d_sets = nullptr;
+    tmc::BatchChunk* d_chunks = nullptr;
+    unsigned long long* d_tally = nullptr;
+    unsigned long long* d_done = nullptr;
+    unsigned long long* d_counters = nullptr;
+    unsigned long long* h_tally = nullptr; /* pinned */
+    unsigned long long* h_counters = nullptr; /* pinned, 4 words */
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    tmc::BatchArgs args;
+};
+
+namespace {
+
+void plan_free(tmc_batch_plan* p)
+{
+    if (!p)
+        return;
+    if (p->exec)
+        (void)hipGraphExecDestroy(p->exec);
+    if (p->graph)
+        (void)hipGraphDestroy(p->graph);
+    if (p->d_sets)
+        (void)hipFree(p->d_sets);
+    if (p->d_chunks)
+        (void)hipFree(p->d_chunks);
+    if (p->d_tally)
+        (void)hipFree(p->d_tally);
+    if (p->d_done)
+        (void)hipFree(p->d_done);
+    if (p->d_counters)
+        (void)hipFree(p->d_counters);
+    if (p->h_tally)
+        (void)hipHostFree(p->h_tally);
+    if (p->h_counters)
+        (void)hipHostFree(p->h_counters);
+    delete p;
+}
+
+/* Enqueues clear + walk + copies on `stream`. */
+hipError_t enqueue_run(tmc_batch_plan* p, hipStream_t stream)
+{
+    hipError_t e = tmc::launch_batch_clear(p->d_tally, p->tally_words, p->d_done, p->nsets, p->d_counters, stream);
+    if (e != hipSuccess)
+        return e;
+    if (p->nchunks) {
+        e = tmc::launch_batch(p->sampler, p->args, p->grid, p->block, stream);
+        if (e != hipSuccess)
+            return e;
+    }
+    e = hipMemcpyAsync(p->h_tally, p->d_tally, p->tally_words * sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess)
+        return e;
+    return hipMemcpyAsync(p->h_counters, p->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
+}
+
+int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, const tmc_batch_options* o,
+                       tmc_batch_plan** out)
+{
+    if (o->sampler > TMC_SAMPLER_REJECTION)
+        return TMC_E_INVALID;
+    if (o->block && (o->block < 64u || o->block > 1024u || o->block % 64u))
+        return TMC_E_INVALID;
+    const uint32_t copies = o->copies ? o->copies : 1u;
+    if (copies != 1u && copies != 2u)
+        return TMC_E_INVALID;
+    if (o->waves_per_simd > 8u)
+        return TMC_E_INVALID;
+
+    tmc_batch_plan* p = new (std::nothrow) tmc_batch_plan;
+    if (!p)
+        return TMC_E_NOMEM;
+    p->nsets = nsets;
+    p->sampler = o->sampler;
+    p->copies = copies;
+    p->use_graph = o->use_graph != 0;
+
+    /* set table */
+    std::vector<tmc::BatchSet> hs(nsets);
+    std::vector<double> L(nsets);
+    uint64_t off = 0;
+    double work = 0.0;
+    uint32_t shells_max = 1;
+    uint64_t kcap = o->max_interactions ? o->max_interactions : 0xFFFFFFFFull;
+    if (kcap > 0xFFFFFFFFull)
+        kcap = 0xFFFFFFFFull;
+    for (uint32_t i = 0; i < nsets; ++i) {
+        const tmc_params& q = sets[i].params;
+        tmc::BatchSet& d = hs[i];
+        std::memset(&d, 0, sizeof d);
+        d.seed = sets[i].seed;
+        d.first_photon = sets[i].first_photon;
+        d.tally_offset = off;
+        d.albedo = tmc::derived_albedo(&q);
+        d.spm = tmc::derived_spm(&q);
+        d.shell_max = (float)(q.shells - 1);
+        d.shells = q.shells;
+        d.kcap = (uint32_t)kcap;
+        off += 4 * (uint64_t)q.shells;
+        shells_max = std::max(shells_max, q.shells);
+        L[i] = tmc::expected_interactions(&q, kcap);
+        work += L[i] * (double)q.photons;
+        p->photons += q.photons;
+    }
+    p->tally_words = off;
+    p->shells_max = shells_max;
+    p->expected = work;
+
+    /* launch shape: the requested block, or the largest of 256/128/64 whose
+     * per-wave LDS slots fit on a CU */
+    int wg_per_cu = 0;
+    uint32_t block = 0;
+    std::vector<uint32_t> cand;
+    if (o->block)
+        cand.push_back(o->block);
+    else
+        cand = { 256u, 128u, 64u };
+    for (uint32_t b : cand) {
+        int n = 0;
+        if (tmc::batch_lds_bytes(b, shells_max, copies) <= 160u * 1024u
+            && tmc::batch_occupancy(p->sampler, b, shells_max, copies, &n) != hipSuccess) {
+            plan_free(p);
+            return TMC_E_HIP;
+        }
+        if (n > 0) {
+            wg_per_cu = n;
+            block = b;
+            break;
+        }
+    }
+    if (wg_per_cu <= 0) {
+        plan_free(p);
+        return TMC_E_INVALID; /* LDS slots of shells_max shells do not fit */
+    }
+    p->block = block;
+    const uint32_t waves_per_wg = block / 64u;
+    if (o->waves_per_simd) {
+        const uint32_t want = std::max(1u, o->waves_per_simd * 4u / waves_per_wg);
+        wg_per_cu = std::min<int>(wg_per_cu, (int)want);
+    }
+    p->grid = (uint32_t)wg_per_cu * (uint32_t)c->compute_units;
+    p->wps = (uint32_t)wg_per_cu * waves_per_wg / 4u;
+    const double total_waves = (double)p->grid * waves_per_wg;
+
+    /* chunk queue */
+    uint64_t cw = o->chunk_work;
+    if (!cw)
+        cw = (uint64_t)std::max(65536.0, work / (16.0 * total_waves));
+    p->chunk_work = cw;
+    struct Item {
+        tmc::BatchChunk ch;
+        double dur;
+    };
+    std::vector<Item> items;
+    for (uint32_t i = 0; i < nsets; ++i) {
+        const uint64_t N = sets[i].params.photons;
+        if (!N)
+            continue;
+        double per_lane = std::ceil((double)cw / (64.0 * L[i]));
+        per_lane = std::min(std::max(per_lane, 1.0), (double)(1u << 24));
+        uint64_t C = 64u * (uint64_t)per_lane;
+        /* keep the queue below 2^24 chunks */
+        const uint64_t min_c = (N >> 24) + 1;
+        if (C < min_c)
+            C = (min_c + 63) / 64 * 64;
+        if (C > (1ull << 31))
+            C = 1ull << 31;
+        for (uint64_t first = 0; first < N; first += C) {
+            const uint64_t cnt = std::min(C, N - first);
+            items.push_back({ { i, (uint32_t)cnt, first }, (double)((cnt + 63) / 64) * L[i] });
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.dur > y.dur; });
+    p->nchunks = items.size();
+    std::vector<tmc::BatchChunk> hc(items.size());
+    for (size_t i = 0; i < items.size(); ++i)
+        hc[i] = items[i].ch;
+
+    /* device memory */
+    if (hipMalloc((void**)&p->d_sets, std::max<size_t>(1, nsets) * sizeof(tmc::BatchSet)) != hipSuccess
+        || hipMalloc((void**)&p->d_chunks, std::max<size_t>(1, hc.size()) * sizeof(tmc::BatchChunk)) != hipSuccess
+        || hipMalloc((void**)&p->d_tally, std::max<uint64_t>(1, p->tally_words) * sizeof(uint64_t)) != hipSuccess
+        || hipMalloc((void**)&p->d_done, std::max<size_t>(1, nsets) * sizeof(uint64_t)) != hipSuccess
+        || hipMalloc((void**)&p->d_counters, 4 * sizeof(uint64_t)) != hipSuccess
+        || hipHostMalloc((void**)&p->h_tally, std::max<uint64_t>(1, p->tally_words) * sizeof(uint64_t), 0)
+            != hipSuccess
+        || hipHostMalloc((void**)&p->h_counters, 4 * sizeof(uint64_t), 0) != hipSuccess) {
+        plan_free(p);
+        return TMC_E_NOMEM;
+    }
+    if ((nsets && hipMemcpy(p->d_sets, hs.data(), nsets * sizeof(tmc::BatchSet), hipMemcpyHostToDevice) != hipSuccess)
+        || (!hc.empty()
+            && hipMemcpy(p->d_chunks, hc.data(), hc.size() * sizeof(tmc::BatchChunk), hipMemcpyHostToDevice)
+                != hipSuccess)) {
+        plan_free(p);
+        return TMC_E_HIP;
+    }
+
+    std::memset(&p->args, 0, sizeof p->args);
+    p->args.sets = p->d_sets;
+    p->args.chunks = p->d_chunks;
+    p->args.nchunks = p->nchunks;
+    p->args.shells_max = shells_max;
+    p->args.copies = copies;
+    /* a wave alone adds to its slot: 64 lanes x flush_interval deposits of
+     * at most 2^43 (heat2 image of w = 1) per entry stay below 2^63 */
+    p->args.flush_interval = 1u << 14;
+    p->args.counters = p->d_counters;
+    p->args.tally = p->d_tally;
+    p->args.done = p->d_done;
+
+    if (p->use_graph) {
+        hipStream_t s = c->stream;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            plan_free(p);
+            return TMC_E_HIP;
+        }
+        const hipError_t e1 = enqueue_run(p, s);
+        const hipError_t e2 = hipStreamEndCapture(s, &p->graph);
+        if (e1 != hipSuccess || e2 != hipSuccess
+            || hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess) {
+            plan_free(p);
+            return TMC_E_HIP;
+        }
+    }
+    *out = p;
+    return TMC_OK;
+}
+
+int plan_run_locked(tmc::Context* c, tmc_batch_plan* p, uint64_t* tallies, tmc_batch_stats* st)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    if (hipEventRecord(c->ev_start, c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    if (p->use_graph) {
+        if (hipGraphLaunch(p->exec, c->stream) != hipSuccess)
+            return TMC_E_HIP;
+    } else if (enqueue_run(p, c->stream) != hipSuccess) {
+        return TMC_E_HIP;
+    }
+    if (hipEventRecord(c->ev_stop, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c->ev_start, c->ev_stop) != hipSuccess)
+        return TMC_E_HIP;
+    if (tallies)
+        std::memcpy(tallies, p->h_tally, p->tally_words * sizeof(uint64_t));
+    if (st) {
+        st->sets = p->nsets;
+        st->photons = p->photons;
+        st->interactions = p->h_counters[1];
+        st->truncated = p->h_counters[2];
+        st->chunks = p->nchunks;
+        st->chunk_work = p->chunk_work;
+        st->expected_interactions = p->expected;
+        st->kernel_ms = ms;
+        st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        st->grid = p->grid;
+        st->block = p->block;
+        st->copies = p->copies;
+        st->waves_per_simd = p->wps;
+    }
+    return TMC_OK;
+}
+
+int check_sets(const tmc_set* sets, uint32_t nsets)
+{
+    if (!sets && nsets)
+        return TMC_E_INVALID;
+    for (uint32_t i = 0; i < nsets; ++i) {
+        const int rc = tmc::check_params(&sets[i].params);
+        if (rc != TMC_OK)
+            return rc;
+        if (sets[i].params.photons > UINT64_MAX - sets[i].first_photon)
+            return TMC_E_RANGE;
+    }
+    return TMC_OK;
+}
+
+} // namespace
+
+TMC_EXPORT void tmc_batch_options_default(tmc_batch_options* opt)
+{
+    if (opt)
+        std::memset(opt, 0, sizeof *opt);
+}
+
+TMC_EXPORT uint64_t tmc_batch_tally_words(const tmc_set* sets, uint32_t nsets)
+{
+    uint64_t w = 0;
+    for (uint32_t i = 0; sets && i < nsets; ++i)
+        w += 4 * (uint64_t)sets[i].params.shells;
+    return w;
+}
+
+TMC_EXPORT int tmc_batch_plan_create(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* opt,
+                                     tmc_batch_plan** plan)
+{
+    if (!plan)
+        return TMC_E_INVALID;
+    *plan = nullptr;
+    int rc = check_sets(sets, nsets);
+    if (rc != TMC_OK)
+        return rc;
+    tmc_batch_options o;
+    if (opt)
+        o = *opt;
+    else
+        tmc_batch_options_default(&o);
+    std::lock_guard<std::mutex> lock(tmc::api_mutex());
+    tmc::Context* c = nullptr;
+    rc = tmc::acquire_context(&c);
+    if (rc != TMC_OK)
+        return rc;
+    return plan_create_locked(c, sets, nsets, &o, plan);
+}
+
+TMC_EXPORT int tmc_batch_plan_run(tmc_batch_plan* plan, uint64_t* tallies, tmc_batch_stats* stats)
+{
+    if (!plan)
+        return TMC_E_INVALID;
+    std::lock_guard<std::mutex> lock(tmc::api_mutex());
+    tmc::Context* c = nullptr;
+    const int rc = tmc::acquire_context(&c);
+    if (rc != TMC_OK)
+        return rc;
+    return plan_run_locked(c, plan, tallies, stats);
+}
+
+TMC_EXPORT void tmc_batch_plan_destroy(tmc_batch_plan* plan)
+{
+    if (!plan)
+        return;
+    std::lock_guard<std::mutex> lock(tmc::api_mutex());
+    plan_free(plan);
+}
+
+TMC_EXPORT int tmc_batch_run(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* opt, uint64_t* tallies,
+                             tmc_batch_stats* stats)
+{
+    tmc_batch_plan* plan = nullptr;
+    int rc = tmc_batch_plan_create(sets, nsets, opt, &plan);
+    if (rc != TMC_OK)
+        return rc;
+    rc = tmc_batch_plan_run(plan, tallies, stats);
+    tmc_batch_plan_destroy(plan);
+    return rc;
+}
+
+TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* base,
+                                  uint32_t sample_divisor, tmc_batch_options* best)
+{
+    if (!best)
+        return TMC_E_INVALID;
+    int rc = check_sets(sets, nsets);
+    if (rc != TMC_OK)
+        return rc;
+    tmc_batch_options b0;
+    if (base)
+        b0 = *base;
+    else
+        tmc_batch_options_default(&b0);
+    b0.use_graph = 0;
+    if (sample_divisor == 0)
+        sample_divisor = 1;
+    std::vector<tmc_set> sample(sets, sets + nsets);
+    for (auto& s : sample)
+        if (s.params.photons)
+            s.params.photons = std::max<uint64_t>(64, s.params.photons / sample_divisor);
+
+    /* the automatic chunk work of the full batch on the default shape */
+    tmc_batch_plan* probe = nullptr;
+    rc = tmc_batch_plan_create(sets, nsets, &b0, &probe);
+    if (rc != TMC_OK)
+        return rc;
+    const uint64_t auto_cw = probe->chunk_work;
+    const double full_work = probe->expected;
+    tmc_batch_plan_destroy(probe);
+    double sample_work = 0.0;
+    {
+        tmc_batch_plan* sp = nullptr;
+        rc = tmc_batch_plan_create(sample.data(), nsets, &b0, &sp);
+        if (rc != TMC_OK)
+            return rc;
+        sample_work = sp->expected;
+        tmc_batch_plan_destroy(sp);
+    }
+    /* chunk work is tuned as a multiple of the automatic value, scaled to the
+     * sample so that the sample sees the same chunk count per wave */
+    const double scale = full_work > 0 ? sample_work / full_work : 1.0;
+
+    auto time_of = [&](const tmc_batch_options& o, double mult) -> double {
+        tmc_batch_options t = o;
+        t.chunk_work = (uint64_t)std::max(4096.0, mult * (double)auto_cw * scale);
+        tmc_batch_plan* plan = nullptr;
+        if (tmc_batch_plan_create(sample.data(), nsets, &t, &plan) != TMC_OK)
+            return 1e300;
+        tmc_batch_stats st;
+        double best_ms = 1e300;
+        for (int rep = 0; rep < 2; ++rep) /* first run warms the code object */
+            if (tmc_batch_plan_run(plan, nullptr, &st) == TMC_OK && rep > 0)
+                best_ms = st.kernel_ms;
+        tmc_batch_plan_destroy(plan);
+        return best_ms;
+    };
+
+    tmc_batch_options cur = b0;
+    double cur_mult = 1.0;
+    double cur_ms = time_of(cur, cur_mult);
+    for (double m : { 0.25, 4.0 }) {
+        const double ms = time_of(cur, m);
+        if (ms < cur_ms) {
+            cur_ms = ms;
+            cur_mult = m;
+        }
+    }
+    for (uint32_t copies : { 2u }) {
+        tmc_batch_options t = cur;
+        t.copies = copies;
+        const double ms = time_of(t, cur_mult);
+        if (ms < cur_ms) {
+            cur_ms = ms;
+            cur = t;
+        }
+    }
+    for (uint32_t block : { 64u, 512u }) {
+        tmc_batch_options t = cur;
+        t.block = block;
+        const double ms = time_of(t, cur_mult);
+        if (ms < cur_ms) {
+            cur_ms = ms;
+            cur = t;
+        }
+    }
+    for (uint32_t wps : { 6u, 4u }) {
+        tmc_batch_options t = cur;
+        t.waves_per_simd = wps;
+        const double ms = time_of(t, cur_mult);
+        if (ms < cur_ms) {
+            cur_ms = ms;
+            cur = t;
+        }
+    }
+    if (cur_ms >= 1e300)
+        return TMC_E_HIP;
+    *best = cur;
+    best->chunk_work = (uint64_t)std::max(4096.0, cur_mult * (double)auto_cw);
+    best->use_graph = base ? base->use_graph : 0;
+    return TMC_OK;
+}

@@ -1,0 +1,54 @@
+/* Device-side layout of a batched run (set table, chunk queue) shared by the
+ * multi-set kernel (tmc_batch.hip) and its host side (tmc_batch_api.cpp). */
+#ifndef TMC_BATCH_INTERNAL_H
+#define TMC_BATCH_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tmc {
+
+/* One parameter set, as the kernel reads it (64 bytes). */
+struct BatchSet {
+    uint64_t seed; /* Philox key */
+    uint64_t first_photon; /* stream index of the set's photon 0 */
+    uint64_t tally_offset; /* word offset of the set's 4*shells tally words */
+    float albedo; /* binary32, photon.c:8 */
+    float spm; /* binary32, photon.c:9 */
+    float shell_max; /* (float)(shells - 1) */
+    uint32_t shells;
+    uint32_t kcap; /* stop a photon after kcap absorptions */
+    uint32_t pad[5];
+};
+static_assert(sizeof(BatchSet) == 64, "BatchSet layout");
+
+/* A contiguous photon range of one set (16 bytes). */
+struct BatchChunk {
+    uint32_t set;
+    uint32_t count; /* photons, >= 1 */
+    uint64_t first; /* offset within the set */
+};
+static_assert(sizeof(BatchChunk) == 16, "BatchChunk layout");
+
+struct BatchArgs {
+    const BatchSet* sets;
+    const BatchChunk* chunks;
+    uint64_t nchunks;
+    uint32_t shells_max; /* LDS slot size in shells */
+    uint32_t copies; /* tally copies per wave (1 or 2) */
+    uint32_t flush_interval; /* wave steps between slot flushes */
+    uint32_t pad;
+    unsigned long long* counters; /* [0] chunk cursor [1] interactions [2] truncated */
+    unsigned long long* tally; /* tally pool */
+    unsigned long long* done; /* per set: photons completed and flushed */
+};
+
+size_t batch_lds_bytes(uint32_t block, uint32_t shells_max, uint32_t copies);
+hipError_t launch_batch(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
+hipError_t launch_batch_clear(unsigned long long* tally, uint64_t tally_words, unsigned long long* done,
+                              uint32_t nsets, unsigned long long* counters, hipStream_t stream);
+hipError_t batch_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max, uint32_t copies, int* wg_per_cu);
+
+} // namespace tmc
+
+#endif

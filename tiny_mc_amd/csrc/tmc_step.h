@@ -1,0 +1,112 @@
+/* One absorption event of the photon walk in FP32 (photon.c:21-49), shared
+ * by the single-run and multi-set kernels.  The operation order here is the
+ * specification the host mirror (oracle) reproduces bit for bit. */
+#ifndef TMC_STEP_H
+#define TMC_STEP_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tmc_math.h"
+#include "tmc_philox.h"
+
+namespace tmc {
+
+/* Lane state of one photon (photon.c:11-18 plus its stream position). */
+struct Photon {
+    float x, y, z, u, v, w, weight;
+    uint32_t k; /* absorptions so far = Philox counter word 0 */
+    uint32_t plo, phi; /* photon index = Philox counter words 2-3 */
+};
+
+/* Physics of the photon's parameter set. */
+struct Physics {
+    float albedo; /* mu_s / (mu_s + mu_a) in binary32 (photon.c:8) */
+    float spm; /* shells per mean free path (photon.c:9) */
+    float shell_max; /* (float)(shells - 1) */
+    uint32_t last_shell; /* shells - 1 */
+    uint32_t key0, key1; /* Philox key = 64-bit seed */
+};
+
+__device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon)
+{
+    p.x = p.y = p.z = 0.0f;
+    p.u = p.v = 0.0f;
+    p.w = 1.0f;
+    p.weight = 1.0f;
+    p.k = 0;
+    p.plo = (uint32_t)photon;
+    p.phi = (uint32_t)(photon >> 32);
+}
+
+/* Walks one absorption event of p, adding its deposit to the fixed-point
+ * shell sums lw / lw2 (LDS).  Returns false when the roulette kills the
+ * photon. */
+template <uint32_t SAMPLER>
+__device__ __forceinline__ bool step(Photon& p, const Physics& ph, unsigned long long* lw, unsigned long long* lw2)
+{
+    const tmc_u32x4 r = tmc_philox4x32_10(p.k, 0u, p.plo, p.phi, ph.key0, ph.key1);
+
+    /* move (photon.c:21-24) */
+    const float t = -log32(tmc_u01(r.x0));
+    p.x = __builtin_fmaf(t, p.u, p.x);
+    p.y = __builtin_fmaf(t, p.v, p.y);
+    p.z = __builtin_fmaf(t, p.w, p.z);
+
+    /* absorb (photon.c:26-32) */
+    const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
+    const float f = sqrt_rn(r2) * ph.spm;
+    const uint32_t shell = f < ph.shell_max ? (uint32_t)f : ph.last_shell;
+    atomicAdd(&lw[shell], (unsigned long long)fx_heat(p.weight));
+    atomicAdd(&lw2[shell], (unsigned long long)fx_heat2(p.weight));
+    p.weight = p.weight * ph.albedo;
+
+    /* new direction (photon.c:34-43) */
+    if (SAMPLER == 1u) {
+        uint32_t w0 = r.x1, w1 = r.x2;
+        tmc_u32x4 blk = { 0u, 0u, 0u, 0u };
+        float xi1, xi2, tt;
+        uint32_t att = 0;
+        for (;;) {
+            xi1 = __builtin_fmaf(2.0f, tmc_u01(w0), -1.0f);
+            xi2 = __builtin_fmaf(2.0f, tmc_u01(w1), -1.0f);
+            tt = __builtin_fmaf(xi2, xi2, xi1 * xi1);
+            ++att;
+            if ((tt <= 1.0f && tt > 0.0f) || att >= 64u)
+                break;
+            if (att & 1u) {
+                blk = tmc_philox4x32_10(p.k, (att + 1u) >> 1, p.plo, p.phi, ph.key0, ph.key1);
+                w0 = blk.x0;
+                w1 = blk.x1;
+            } else {
+                w0 = blk.x2;
+                w1 = blk.x3;
+            }
+        }
+        p.u = __builtin_fmaf(2.0f, tt, -1.0f);
+        const float s = 2.0f * sqrt_rn(1.0f - tt);
+        p.v = xi1 * s;
+        p.w = xi2 * s;
+    } else {
+        const float mu = __builtin_fmaf(2.0f, tmc_u01(r.x1), -1.0f);
+        const float s = sqrt_rn(__builtin_fmaf(-mu, mu, 1.0f));
+        float sn, cs;
+        sincos2pi32(tmc_u01(r.x2), &sn, &cs);
+        p.u = mu;
+        p.v = s * cs;
+        p.w = s * sn;
+    }
+    ++p.k;
+
+    /* roulette (photon.c:45-49) */
+    if (p.weight < 0.001f) {
+        if (tmc_u01(r.x3) > 0.1f)
+            return false;
+        p.weight = p.weight / 0.1f;
+    }
+    return true;
+}
+
+} // namespace tmc
+
+#endif

@@ -24,25 +24,13 @@
  * No MFMA: the walk is a scalar, transcendental-heavy loop with no dense
  * contraction; it is bound by VALU issue.
  */
+#include "tmc_device.h"
 #include "tmc_internal.h"
 #include "tmc_math.h"
 #include "tmc_philox.h"
+#include "tmc_step.h"
 
 namespace tmc {
-
-__device__ __forceinline__ uint64_t rfl64(uint64_t v)
-{
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ void add128(unsigned long long* p, unsigned long long v)
-{
-    const unsigned long long old = atomicAdd(p, v);
-    if (old + v < old)
-        atomicAdd(p + 1, 1ull);
-}
 
 /* Moves this workgroup's LDS tallies into the global 128-bit sums.  Safe
  * while other waves of the workgroup keep adding: each entry is taken by an
@@ -80,17 +68,18 @@ __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
     uint64_t wave_interactions = 0, wave_truncated = 0;
     uint32_t since_flush = 0;
 
+    const Physics phys = { a.albedo, a.spm, a.shell_max, a.shells - 1, key0, key1 };
+
     /* lane state: one photon (photon.c:11-18) */
     bool alive = false;
-    float x = 0.0f, y = 0.0f, z = 0.0f, u = 0.0f, v = 0.0f, w = 1.0f, weight = 1.0f;
-    uint32_t k = 0, plo = 0, phi = 0;
+    Photon ph;
+    launch_photon(ph, 0);
 
     for (;;) {
         const uint64_t dead = __ballot(!alive);
         if (dead != 0 && !exhausted) {
             const uint32_t ndead = (uint32_t)__popcll(dead);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(dead >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)dead, 0u));
+            const uint32_t rank = mask_rank(dead);
             const uint64_t avail = pool_end - pool_next;
             uint64_t nb = 0, ne = 0;
             if (avail < ndead) {
@@ -116,14 +105,7 @@ __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
                     got = true;
                 }
                 if (got) {
-                    const uint64_t photon = a.first_photon + idx;
-                    plo = (uint32_t)photon;
-                    phi = (uint32_t)(photon >> 32);
-                    x = y = z = 0.0f;
-                    u = v = 0.0f;
-                    w = 1.0f;
-                    weight = 1.0f;
-                    k = 0;
+                    launch_photon(ph, a.first_photon + idx);
                     alive = true;
                 }
             }
@@ -142,67 +124,8 @@ __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
 
         bool truncated = false;
         if (alive) {
-            const tmc_u32x4 r = tmc_philox4x32_10(k, 0u, plo, phi, key0, key1);
-
-            /* move (photon.c:21-24) */
-            const float t = -log32(tmc_u01(r.x0));
-            x = __builtin_fmaf(t, u, x);
-            y = __builtin_fmaf(t, v, y);
-            z = __builtin_fmaf(t, w, z);
-
-            /* absorb (photon.c:26-32) */
-            const float r2 = __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
-            const float f = sqrt_rn(r2) * a.spm;
-            const uint32_t shell = f < a.shell_max ? (uint32_t)f : a.shells - 1;
-            atomicAdd(&lw[shell], (unsigned long long)fx_heat(weight));
-            atomicAdd(&lw2[shell], (unsigned long long)fx_heat2(weight));
-            weight = weight * a.albedo;
-
-            /* new direction (photon.c:34-43) */
-            if (SAMPLER == 1u) {
-                uint32_t w0 = r.x1, w1 = r.x2;
-                tmc_u32x4 blk = { 0u, 0u, 0u, 0u };
-                float xi1, xi2, tt;
-                uint32_t att = 0;
-                for (;;) {
-                    xi1 = __builtin_fmaf(2.0f, tmc_u01(w0), -1.0f);
-                    xi2 = __builtin_fmaf(2.0f, tmc_u01(w1), -1.0f);
-                    tt = __builtin_fmaf(xi2, xi2, xi1 * xi1);
-                    ++att;
-                    if ((tt <= 1.0f && tt > 0.0f) || att >= 64u)
-                        break;
-                    if (att & 1u) {
-                        blk = tmc_philox4x32_10(k, (att + 1u) >> 1, plo, phi, key0, key1);
-                        w0 = blk.x0;
-                        w1 = blk.x1;
-                    } else {
-                        w0 = blk.x2;
-                        w1 = blk.x3;
-                    }
-                }
-                u = __builtin_fmaf(2.0f, tt, -1.0f);
-                const float s = 2.0f * sqrt_rn(1.0f - tt);
-                v = xi1 * s;
-                w = xi2 * s;
-            } else {
-                const float mu = __builtin_fmaf(2.0f, tmc_u01(r.x1), -1.0f);
-                const float s = sqrt_rn(__builtin_fmaf(-mu, mu, 1.0f));
-                float sn, cs;
-                sincos2pi32(tmc_u01(r.x2), &sn, &cs);
-                u = mu;
-                v = s * cs;
-                w = s * sn;
-            }
-            ++k;
-
-            /* roulette (photon.c:45-49) */
-            if (weight < 0.001f) {
-                if (tmc_u01(r.x3) > 0.1f)
-                    alive = false;
-                else
-                    weight = weight / 0.1f;
-            }
-            if (alive && k >= a.kcap) {
+            alive = step<SAMPLER>(ph, phys, lw, lw2);
+            if (alive && ph.k >= a.kcap) {
                 alive = false;
                 truncated = true;
             }

@@ -115,6 +115,11 @@ void tiny_mc_gpu_shutdown(void);
 /* Device name (gcnArchName) and compute-unit count of the context's device. */
 int tiny_mc_gpu_device_info(char* name, size_t cap, int* compute_units);
 
+/* Expected absorptions per photon for p (the binary32 weight cadence of
+ * photon.c:32,45-49 and the roulette survival probability): the estimate the
+ * batched and served modes schedule with.  Host only. */
+double tiny_mc_gpu_expected_interactions(const tmc_params* p);
+
 /* Human-readable name of a tmc_status. */
 const char* tiny_mc_gpu_strerror(int status);
 
@@ -194,6 +199,73 @@ int tmc_batch_run(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* 
  * the fastest options to *best (starting from *base). */
 int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* base,
                        uint32_t sample_divisor, tmc_batch_options* best);
+
+/* ------------------------------------------------------------------------
+ * Serving: a request loop over a resident persistent kernel.
+ *
+ * No counterpart in the reference.  Requests (one tmc_set each) are admitted
+ * while tally arena, request slots and chunk ring have room, in arrival
+ * order; their chunks are published round-robin into a ring in
+ * host-coherent memory that the running kernel polls, so requests that
+ * arrive while others are on the GPU join the same launch.  Every result is
+ * bit-identical to tiny_mc_gpu_run_ex of the same set alone.  A server owns
+ * a host thread and two default-priority streams on the context's device.
+ * ------------------------------------------------------------------------ */
+
+#define TMC_PENDING 1 /* positive status: the request has not finished */
+
+typedef struct tmc_serve_options {
+    uint32_t sampler; /* TMC_SAMPLER_* for every request */
+    uint32_t max_shells; /* largest shell count of a request; 0 = 401 */
+    uint32_t max_requests; /* resident requests (slots), <= 65536; 0 = 4096 */
+    uint32_t block; /* threads per workgroup; 0 = 256 */
+    uint64_t tally_words; /* device tally arena in 64-bit words; 0 = max_requests * 4 * max_shells */
+    uint64_t ring_chunks; /* chunk ring entries, power of two; 0 = 2^20 */
+    uint64_t chunk_work; /* target absorptions per chunk; 0 = 2^20 */
+    uint32_t max_photons_per_lane; /* cap on photons per lane in a chunk; 0 = 16 */
+    uint32_t idle_ms; /* the kernel leaves after this long without work; 0 = 50 */
+} tmc_serve_options;
+
+typedef struct tmc_request_stats {
+    uint64_t photons;
+    uint64_t interactions;
+    double submit_s; /* seconds since the server opened */
+    double admit_s;
+    double done_s;
+    double latency_ms; /* done - submit */
+} tmc_request_stats;
+
+typedef struct tmc_serve_stats {
+    uint64_t submitted, admitted, completed;
+    uint64_t photons_completed, interactions_completed;
+    uint64_t kernel_launches;
+    uint64_t chunks_published;
+    double elapsed_s;
+} tmc_serve_stats;
+
+typedef struct tmc_server tmc_server;
+
+void tmc_serve_options_default(tmc_serve_options* opt);
+
+int tmc_serve_open(const tmc_serve_options* opt, tmc_server** server);
+
+/* Queues a request; returns its id.  TMC_E_INVALID for bad parameters or
+ * more shells than max_shells, TMC_E_NOMEM if its tally can never fit. */
+int tmc_serve_submit(tmc_server* server, const tmc_set* request, uint64_t* id);
+
+/* TMC_OK and the request's tally (4 * shells words, layout of
+ * tiny_mc_gpu_run_ex) once it is done — the result is handed over once —,
+ * TMC_PENDING before, TMC_E_INVALID for an unknown id. */
+int tmc_serve_poll(tmc_server* server, uint64_t id, uint64_t* tally, tmc_request_stats* stats);
+
+/* As tmc_serve_poll, blocking up to timeout_s seconds (< 0: no limit). */
+int tmc_serve_wait(tmc_server* server, uint64_t id, double timeout_s, uint64_t* tally, tmc_request_stats* stats);
+
+int tmc_serve_get_stats(tmc_server* server, tmc_serve_stats* stats);
+
+/* Finishes every submitted request, stops the kernel, frees the server.
+ * Results not yet polled are discarded. */
+int tmc_serve_close(tmc_server* server);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,105 @@
+"""Serving (continuous batching on a resident persistent kernel): each
+result equals the same request run alone, bit for bit, whether requests
+arrive together, staggered while others run, or wait for memory."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_requests(tmc, n, seed):
+    reqs = tmc.serve_load.draw_requests(n, seed=seed)
+    # keep the suite quick: shrink the heavy tail
+    return [(tmc.Params(p.mu_a, p.mu_s, p.shells, p.microns_per_shell, max(64, p.photons // 64)), s)
+            for p, s in reqs]
+
+
+def alone(tmc, p, s, sampler=0):
+    return tmc.run_raw(p, s, sampler=sampler)[0]
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_alone_equals_in_the_mix(tmc, sampler):
+    reqs = mixed_requests(tmc, 40, seed=3 + sampler)
+    with tmc.Server(sampler=sampler) as srv:
+        ids = []
+        for i, (p, s) in enumerate(reqs):
+            ids.append(srv.submit(p, s))
+            if i % 10 == 9:
+                time.sleep(0.02)  # later requests arrive while earlier ones run
+        got = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    for (p, s), (t, rs) in zip(reqs, got):
+        np.testing.assert_array_equal(t, alone(tmc, p, s, sampler))
+        assert rs.photons == p.photons and rs.latency_ms >= 0 and rs.done_s >= rs.admit_s >= rs.submit_s
+    assert st["completed"] == len(reqs)
+
+
+def test_admission_waits_for_memory(tmc):
+    """A tally arena with room for two 101-shell requests at a time: every
+    request still completes, in arrival order of admission, with exact
+    results; a request whose tally can never fit is refused."""
+    reqs = [(tmc.Params(photons=20000 + 1000 * i), 50 + i) for i in range(8)]
+    with tmc.Server(max_requests=2, tally_words=2 * 4 * 101, max_shells=101) as srv:
+        ids = [srv.submit(p, s) for p, s in reqs]
+        got = [srv.wait(i) for i in ids]
+        with pytest.raises(tmc.TmcError) as e:
+            srv.submit(tmc.Params(shells=401), 1)
+        assert e.value.status == -1  # more shells than max_shells
+    admits = [rs.admit_s for _, rs in got]
+    assert admits == sorted(admits)
+    for (p, s), (t, _) in zip(reqs, got):
+        np.testing.assert_array_equal(t, alone(tmc, p, s))
+    with tmc.Server(max_requests=2, tally_words=100, max_shells=101) as srv:
+        with pytest.raises(tmc.TmcError) as e:
+            srv.submit(tmc.Params(), 1)
+        assert e.value.status == -4
+
+
+def test_zero_photons_bad_params_and_poll(tmc):
+    with tmc.Server() as srv:
+        rid = srv.submit(tmc.Params(photons=0), 1)
+        t, rs = srv.wait(rid)
+        assert not t.any() and rs.photons == 0
+        with pytest.raises(tmc.TmcError):
+            srv.submit(tmc.Params(mu_a=0.0), 1)
+        rid = srv.submit(tmc.Params(photons=200000), 7)
+        res = None
+        for _ in range(20000):
+            res = srv.poll(rid)
+            if res is not None:
+                break
+            time.sleep(1e-3)
+        assert res is not None
+        np.testing.assert_array_equal(res[0], alone(tmc, tmc.Params(photons=200000), 7))
+
+
+def test_kernel_relaunch_after_idle(tmc):
+    with tmc.Server(idle_ms=10) as srv:
+        a = srv.wait(srv.submit(tmc.Params(photons=50000), 1))
+        time.sleep(0.2)  # the kernel leaves after 10 ms without work
+        b = srv.wait(srv.submit(tmc.Params(photons=50000), 2))
+        st = srv.stats()
+    assert st["kernel_launches"] >= 2
+    np.testing.assert_array_equal(a[0], alone(tmc, tmc.Params(photons=50000), 1))
+    np.testing.assert_array_equal(b[0], alone(tmc, tmc.Params(photons=50000), 2))
+
+
+def test_close_finishes_submitted(tmc):
+    srv = tmc.Server()
+    for i in range(5):
+        srv.submit(tmc.Params(photons=100000), i)
+    srv.close()
+
+
+def test_open_loop_load_exact(tmc):
+    """A short Poisson stream: every result exact, latencies recorded."""
+    reqs = mixed_requests(tmc, 60, seed=17)
+    with tmc.Server() as srv:
+        recs, wall = tmc.serve_load.run_open_loop(srv, reqs, rate_per_s=400.0)
+    lat = np.array([r["stats"].latency_ms for r in recs])
+    assert np.all(lat > 0)
+    for (p, s), r in zip(reqs, recs):
+        np.testing.assert_array_equal(r["tally"], alone(tmc, p, s))

@@ -7,7 +7,11 @@ params.h macros become ``Params`` fields and the photon() loop of
 tiny_mc.c:43-49 becomes one ``run`` call.
 """
 from ._lib import (  # noqa: F401
+    PENDING,
     BatchOptions,
+    RequestStats,
+    Server,
+    expected_interactions,
     BatchPlan,
     BatchStats,
     batch_autotune,
@@ -31,5 +35,6 @@ from ._lib import (  # noqa: F401
     shutdown,
     tally_to_heat,
 )
+from . import serve_load  # noqa: F401
 from .sweep import sweep_sets  # noqa: F401
 from .table import format_table, table_rows  # noqa: F401

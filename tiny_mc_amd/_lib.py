@@ -408,3 +408,142 @@ def batch_autotune(sets, base: BatchOptions = None, sample_divisor=16):
            "tmc_batch_autotune")
     return BatchOptions(out.sampler, out.block, out.waves_per_simd, out.copies, out.chunk_work,
                         out.max_interactions, bool(out.use_graph))
+
+
+def expected_interactions(params: Params):
+    """Expected absorptions per photon (host estimate from the weight cadence)."""
+    lib = load()
+    lib.tiny_mc_gpu_expected_interactions.argtypes = [ctypes.POINTER(_Params)]
+    lib.tiny_mc_gpu_expected_interactions.restype = ctypes.c_double
+    c = params._c()
+    return lib.tiny_mc_gpu_expected_interactions(ctypes.byref(c))
+
+
+# ---------------------------------------------------------------------------
+# Serving (tmc_serve_*)
+# ---------------------------------------------------------------------------
+
+PENDING = 1
+
+
+class _ServeOptions(ctypes.Structure):
+    _fields_ = [
+        ("sampler", ctypes.c_uint32),
+        ("max_shells", ctypes.c_uint32),
+        ("max_requests", ctypes.c_uint32),
+        ("block", ctypes.c_uint32),
+        ("tally_words", ctypes.c_uint64),
+        ("ring_chunks", ctypes.c_uint64),
+        ("chunk_work", ctypes.c_uint64),
+        ("max_photons_per_lane", ctypes.c_uint32),
+        ("idle_ms", ctypes.c_uint32),
+    ]
+
+
+class _RequestStats(ctypes.Structure):
+    _fields_ = [
+        ("photons", ctypes.c_uint64),
+        ("interactions", ctypes.c_uint64),
+        ("submit_s", ctypes.c_double),
+        ("admit_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+    ]
+
+
+class _ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("admitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("photons_completed", ctypes.c_uint64),
+        ("interactions_completed", ctypes.c_uint64),
+        ("kernel_launches", ctypes.c_uint64),
+        ("chunks_published", ctypes.c_uint64),
+        ("elapsed_s", ctypes.c_double),
+    ]
+
+
+@dataclass
+class RequestStats:
+    photons: int
+    interactions: int
+    submit_s: float
+    admit_s: float
+    done_s: float
+    latency_ms: float
+
+
+def _bind_serve(lib):
+    if getattr(lib, "_tmc_serve_bound", False):
+        return
+    P = ctypes.POINTER
+    lib.tmc_serve_open.argtypes = [P(_ServeOptions), P(ctypes.c_void_p)]
+    lib.tmc_serve_submit.argtypes = [ctypes.c_void_p, P(_Set), P(ctypes.c_uint64)]
+    lib.tmc_serve_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, P(ctypes.c_uint64), P(_RequestStats)]
+    lib.tmc_serve_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, P(ctypes.c_uint64),
+                                   P(_RequestStats)]
+    lib.tmc_serve_get_stats.argtypes = [ctypes.c_void_p, P(_ServeStats)]
+    lib.tmc_serve_close.argtypes = [ctypes.c_void_p]
+    for n in ("tmc_serve_open", "tmc_serve_submit", "tmc_serve_poll", "tmc_serve_wait", "tmc_serve_get_stats",
+              "tmc_serve_close"):
+        getattr(lib, n).restype = ctypes.c_int
+    lib._tmc_serve_bound = True
+
+
+class Server:
+    """Request loop over a resident persistent kernel (tmc_serve_*)."""
+
+    def __init__(self, sampler=SAMPLER_ANALYTIC, max_shells=0, max_requests=0, block=0, tally_words=0,
+                 ring_chunks=0, chunk_work=0, max_photons_per_lane=0, idle_ms=0):
+        lib = load()
+        _bind_serve(lib)
+        self._lib = lib
+        o = _ServeOptions(sampler, max_shells, max_requests, block, tally_words, ring_chunks, chunk_work,
+                          max_photons_per_lane, idle_ms)
+        h = ctypes.c_void_p()
+        _check(lib.tmc_serve_open(ctypes.byref(o), ctypes.byref(h)), "tmc_serve_open")
+        self._h = h
+        self._shells = {}
+
+    def submit(self, params: Params, seed: int, first_photon: int = 0):
+        s = _Set(params._c(), seed, first_photon)
+        rid = ctypes.c_uint64()
+        _check(self._lib.tmc_serve_submit(self._h, ctypes.byref(s), ctypes.byref(rid)), "tmc_serve_submit")
+        self._shells[rid.value] = params.shells
+        return rid.value
+
+    def _result(self, rc, rid, t, st, what):
+        if rc == PENDING:
+            return None
+        _check(rc, what)
+        self._shells.pop(rid, None)
+        return t, RequestStats(st.photons, st.interactions, st.submit_s, st.admit_s, st.done_s, st.latency_ms)
+
+    def poll(self, rid):
+        t = np.zeros(4 * self._shells[rid], dtype=np.uint64)
+        st = _RequestStats()
+        rc = self._lib.tmc_serve_poll(self._h, rid, _ptr(t, ctypes.c_uint64), ctypes.byref(st))
+        return self._result(rc, rid, t, st, "tmc_serve_poll")
+
+    def wait(self, rid, timeout=-1.0):
+        t = np.zeros(4 * self._shells[rid], dtype=np.uint64)
+        st = _RequestStats()
+        rc = self._lib.tmc_serve_wait(self._h, rid, timeout, _ptr(t, ctypes.c_uint64), ctypes.byref(st))
+        return self._result(rc, rid, t, st, "tmc_serve_wait")
+
+    def stats(self):
+        st = _ServeStats()
+        _check(self._lib.tmc_serve_get_stats(self._h, ctypes.byref(st)), "tmc_serve_get_stats")
+        return {f: getattr(st, f) for f, _ in _ServeStats._fields_}
+
+    def close(self):
+        if self._h:
+            h, self._h = self._h, None
+            _check(self._lib.tmc_serve_close(h), "tmc_serve_close")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -395,6 +395,13 @@ TMC_EXPORT int tiny_mc_gpu_device_info(char* name, size_t cap, int* compute_unit
     return TMC_OK;
 }
 
+TMC_EXPORT double tiny_mc_gpu_expected_interactions(const tmc_params* p)
+{
+    if (tmc::check_params(p) != TMC_OK)
+        return 0.0;
+    return tmc::expected_interactions(p, 0);
+}
+
 TMC_EXPORT const char* tiny_mc_gpu_strerror(int status)
 {
     switch (status) {
@@ -405,6 +412,7 @@ TMC_EXPORT const char* tiny_mc_gpu_strerror(int status)
     case TMC_E_NOMEM: return "out of memory";
     case TMC_E_RANGE: return "photon index range overflows 64 bits";
     case TMC_E_UNSUPPORTED: return "unsupported option";
+    case TMC_PENDING: return "pending";
     default: return "unknown status";
     }
 }

@@ -58,25 +58,144 @@ struct SetState {
     unsigned long long* done;
 };
 
-__device__ __forceinline__ void load_set(SetState& st, const BatchSet* sets, uint32_t set,
-                                         unsigned long long* pool, unsigned long long* done)
+__device__ __forceinline__ uint64_t load_sys(const uint64_t* p)
 {
-    const BatchSet& d = sets[set];
-    st.phys.albedo = d.albedo;
-    st.phys.spm = d.spm;
-    st.phys.shell_max = d.shell_max;
-    st.phys.last_shell = d.shells - 1;
-    st.phys.key0 = (uint32_t)d.seed;
-    st.phys.key1 = (uint32_t)(d.seed >> 32);
-    st.shells = d.shells;
-    st.kcap = d.kcap;
-    st.first_photon = d.first_photon;
-    st.tally = pool + d.tally_offset;
-    st.done = done + set;
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-template <uint32_t SAMPLER>
-__global__ void __launch_bounds__(1024) walk_batch(BatchArgs a)
+/* Loads set `index` into the wave-uniform state.  Batch tables are device
+ * memory written before the launch; serving tables are written by the host
+ * while the kernel runs, so lane 0 reads them with system-scope loads and
+ * broadcasts the words. */
+template <bool SERVE>
+__device__ __forceinline__ void load_set(SetState& st, const BatchArgs& a, uint32_t index, uint32_t lane)
+{
+    uint64_t w[4];
+    const uint64_t* src = (const uint64_t*)(a.sets + index);
+    if (SERVE) {
+        uint64_t v[4] = { 0, 0, 0, 0 };
+        if (lane == 0)
+            for (int i = 0; i < 4; ++i)
+                v[i] = load_sys(src + i);
+        for (int i = 0; i < 4; ++i)
+            w[i] = rfl64(v[i]);
+    } else {
+        for (int i = 0; i < 4; ++i)
+            w[i] = src[i];
+    }
+    /* words: seed, first_photon, tally_offset, (albedo | spm << 32),
+     * then (shell_max | shells << 32), kcap in the next word */
+    uint64_t w4 = 0, w5 = 0;
+    if (SERVE) {
+        uint64_t v4 = 0, v5 = 0;
+        if (lane == 0) {
+            v4 = load_sys(src + 4);
+            v5 = load_sys(src + 5);
+        }
+        w4 = rfl64(v4);
+        w5 = rfl64(v5);
+    } else {
+        w4 = src[4];
+        w5 = src[5];
+    }
+    st.phys.key0 = (uint32_t)w[0];
+    st.phys.key1 = (uint32_t)(w[0] >> 32);
+    st.first_photon = w[1];
+    st.tally = a.tally + w[2];
+    st.phys.albedo = __uint_as_float((uint32_t)w[3]);
+    st.phys.spm = __uint_as_float((uint32_t)(w[3] >> 32));
+    st.phys.shell_max = __uint_as_float((uint32_t)w4);
+    st.shells = (uint32_t)(w4 >> 32);
+    st.phys.last_shell = st.shells - 1;
+    st.kcap = (uint32_t)w5;
+    st.done = a.done + 2 * (uint64_t)index;
+}
+
+/* Next chunk for the wave (wave-uniform).  Batch: the next index of the
+ * fixed queue.  Serving: the next published chunk of the ring, claimed by
+ * compare-and-swap so no index is reserved before it exists; returns false
+ * when the host asked to stop, or after idle_ticks without work, so every
+ * wave leaves. */
+template <bool SERVE>
+__device__ __forceinline__ bool next_chunk(const BatchArgs& a, uint32_t lane, BatchChunk& ch)
+{
+    if (!SERVE) {
+        unsigned long long c = 0;
+        if (lane == 0)
+            c = atomicAdd(a.counters, 1ull);
+        c = rfl64(c);
+        if (c >= a.nchunks)
+            return false;
+        ch = a.chunks[c];
+        return true;
+    }
+    uint64_t got = ~0ull, w0 = 0, w1 = 0;
+    if (lane == 0) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        for (;;) {
+            const uint64_t tail = __hip_atomic_load(&a.ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            unsigned long long c = __hip_atomic_load(a.counters, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c < tail) {
+                /* ring slot c holds chunk c from its publication until the
+                 * cursor is a full wave count past it, so reading it before
+                 * the claim is safe; a lost race discards the read */
+                const uint64_t* slot = (const uint64_t*)(a.chunks + (c & a.ring_mask));
+                w0 = load_sys(slot);
+                w1 = load_sys(slot + 1);
+                if (atomicCAS(a.counters, c, c + 1) == c) {
+                    got = c;
+                    break;
+                }
+                continue;
+            }
+            if (__hip_atomic_load(&a.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+                break;
+            if (__builtin_amdgcn_s_memrealtime() - t0 > a.idle_ticks)
+                break;
+            __builtin_amdgcn_s_sleep(32);
+        }
+    }
+    got = rfl64(got);
+    if (got == ~0ull)
+        return false;
+    w0 = rfl64(w0);
+    w1 = rfl64(w1);
+    ch.set = (uint32_t)w0;
+    ch.count = (uint32_t)(w0 >> 32);
+    ch.first = w1;
+    return true;
+}
+
+/* Flushes the slot into the set's sums, then publishes the photons completed
+ * since the last flush (and their absorptions); when serving, a system-scope
+ * release fence in between keeps the set's counter from running ahead of its
+ * tallies as the host sees them. */
+template <bool SERVE>
+__device__ __forceinline__ void flush_and_publish(unsigned long long* slot, const BatchArgs& a, const SetState& st,
+                                                  uint32_t lane, uint64_t& completed, uint64_t& absorbed)
+{
+    flush_slot(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
+    /* serving: the host reads counters and tallies while the kernel runs;
+     * a batch's are read after the kernel ends, which orders them anyway */
+    if (SERVE)
+        __threadfence_system();
+    if (lane == 0 && (completed || absorbed)) {
+        if (SERVE) {
+            __hip_atomic_fetch_add(st.done, (unsigned long long)completed, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_fetch_add(st.done + 1, (unsigned long long)absorbed, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+        } else {
+            atomicAdd(st.done, (unsigned long long)completed);
+            atomicAdd(st.done + 1, (unsigned long long)absorbed);
+        }
+    }
+    completed = 0;
+    absorbed = 0;
+}
+
+template <uint32_t SAMPLER, bool SERVE>
+__global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
 {
     extern __shared__ unsigned long long lds_slots[];
     const uint32_t lane = __lane_id();
@@ -86,13 +205,14 @@ __global__ void __launch_bounds__(1024) walk_batch(BatchArgs a)
         slot[i] = 0;
     unsigned long long* lw = slot + (a.copies == 2 && lane >= 32 ? 2 * a.shells_max : 0);
     unsigned long long* lw2 = lw + a.shells_max;
+    const uint32_t slot_mask = SERVE ? (1u << TMC_SERVE_SLOT_BITS) - 1u : 0xFFFFFFFFu;
 
     SetState st;
     uint32_t cur_set = 0xFFFFFFFFu;
     uint64_t pool_next = 0, pool_end = 0; /* photon offsets within cur_set */
     bool have_pending = false, queue_done = false;
     BatchChunk pend = { 0u, 0u, 0ull };
-    uint64_t wave_interactions = 0, wave_truncated = 0, completed = 0;
+    uint64_t wave_interactions = 0, wave_truncated = 0, completed = 0, absorbed = 0;
     uint32_t since_flush = 0;
 
     bool alive = false;
@@ -103,35 +223,24 @@ __global__ void __launch_bounds__(1024) walk_batch(BatchArgs a)
         uint64_t dead = __ballot(!alive);
         if (dead != 0) {
             if (pool_next == pool_end && !have_pending && !queue_done) {
-                unsigned long long c = 0;
-                if (lane == 0)
-                    c = atomicAdd(a.counters, 1ull);
-                c = rfl64(c);
-                if (c >= a.nchunks) {
+                BatchChunk ch;
+                if (!next_chunk<SERVE>(a, lane, ch)) {
                     queue_done = true;
+                } else if (ch.set == cur_set) {
+                    pool_next = ch.first;
+                    pool_end = ch.first + ch.count;
                 } else {
-                    const BatchChunk ch = a.chunks[c];
-                    if (ch.set == cur_set) {
-                        pool_next = ch.first;
-                        pool_end = ch.first + ch.count;
-                    } else {
-                        pend = ch;
-                        have_pending = true;
-                    }
+                    pend = ch;
+                    have_pending = true;
                 }
             }
             if (have_pending && dead == ~0ull) {
                 /* drained: hand the finished set's sums over, switch sets */
-                if (cur_set != 0xFFFFFFFFu) {
-                    flush_slot(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
-                    __threadfence();
-                    if (lane == 0 && completed)
-                        atomicAdd(st.done, (unsigned long long)completed);
-                    completed = 0;
-                    since_flush = 0;
-                }
+                if (cur_set != 0xFFFFFFFFu)
+                    flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
+                since_flush = 0;
                 cur_set = pend.set;
-                load_set(st, a.sets, cur_set, a.tally, a.done);
+                load_set<SERVE>(st, a, cur_set & slot_mask, lane);
                 pool_next = pend.first;
                 pool_end = pend.first + pend.count;
                 have_pending = false;
@@ -153,7 +262,9 @@ __global__ void __launch_bounds__(1024) walk_batch(BatchArgs a)
                 break;
             continue;
         }
-        wave_interactions += (uint64_t)__popcll(live);
+        const uint64_t n_live = (uint64_t)__popcll(live);
+        wave_interactions += n_live;
+        absorbed += n_live;
 
         bool truncated = false;
         if (alive) {
@@ -168,19 +279,11 @@ __global__ void __launch_bounds__(1024) walk_batch(BatchArgs a)
 
         if (++since_flush >= a.flush_interval) {
             since_flush = 0;
-            flush_slot(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
-            __threadfence();
-            if (lane == 0 && completed)
-                atomicAdd(st.done, (unsigned long long)completed);
-            completed = 0;
+            flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
         }
     }
-    if (cur_set != 0xFFFFFFFFu) {
-        flush_slot(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
-        __threadfence();
-        if (lane == 0 && completed)
-            atomicAdd(st.done, (unsigned long long)completed);
-    }
+    if (cur_set != 0xFFFFFFFFu)
+        flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
     if (lane == 0) {
         atomicAdd(a.counters + 1, (unsigned long long)wave_interactions);
         if (wave_truncated)
@@ -196,7 +299,7 @@ __global__ void batch_clear(unsigned long long* tally, uint64_t tally_words, uns
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tally_words; i += stride)
         tally[i] = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nsets; i += stride)
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * (uint64_t)nsets; i += stride)
         done[i] = 0;
     if (blockIdx.x == 0 && threadIdx.x < 4)
         counters[threadIdx.x] = 0;
@@ -211,9 +314,19 @@ hipError_t launch_batch(uint32_t sampler, const BatchArgs& a, uint32_t grid, uin
 {
     const size_t lds = batch_lds_bytes(block, a.shells_max, a.copies);
     if (sampler == 1u)
-        hipLaunchKernelGGL(walk_batch<1u>, dim3(grid), dim3(block), lds, stream, a);
+        hipLaunchKernelGGL((walk_multi<1u, false>), dim3(grid), dim3(block), lds, stream, a);
     else
-        hipLaunchKernelGGL(walk_batch<0u>, dim3(grid), dim3(block), lds, stream, a);
+        hipLaunchKernelGGL((walk_multi<0u, false>), dim3(grid), dim3(block), lds, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream)
+{
+    const size_t lds = batch_lds_bytes(block, a.shells_max, a.copies);
+    if (sampler == 1u)
+        hipLaunchKernelGGL((walk_multi<1u, true>), dim3(grid), dim3(block), lds, stream, a);
+    else
+        hipLaunchKernelGGL((walk_multi<0u, true>), dim3(grid), dim3(block), lds, stream, a);
     return hipGetLastError();
 }
 
@@ -228,8 +341,16 @@ hipError_t batch_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max
 {
     const size_t lds = batch_lds_bytes(block, shells_max, copies);
     if (sampler == 1u)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_batch<1u>, (int)block, lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_batch<0u>, (int)block, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_multi<1u, false>, (int)block, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_multi<0u, false>, (int)block, lds);
+}
+
+hipError_t serve_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max, uint32_t copies, int* wg_per_cu)
+{
+    const size_t lds = batch_lds_bytes(block, shells_max, copies);
+    if (sampler == 1u)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_multi<1u, true>, (int)block, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_multi<0u, true>, (int)block, lds);
 }
 
 } // namespace tmc

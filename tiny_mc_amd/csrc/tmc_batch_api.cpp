@@ -219,7 +219,7 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
     if (hipMalloc((void**)&p->d_sets, std::max<size_t>(1, nsets) * sizeof(tmc::BatchSet)) != hipSuccess
         || hipMalloc((void**)&p->d_chunks, std::max<size_t>(1, hc.size()) * sizeof(tmc::BatchChunk)) != hipSuccess
         || hipMalloc((void**)&p->d_tally, std::max<uint64_t>(1, p->tally_words) * sizeof(uint64_t)) != hipSuccess
-        || hipMalloc((void**)&p->d_done, std::max<size_t>(1, nsets) * sizeof(uint64_t)) != hipSuccess
+        || hipMalloc((void**)&p->d_done, std::max<size_t>(1, 2 * (size_t)nsets) * sizeof(uint64_t)) != hipSuccess
         || hipMalloc((void**)&p->d_counters, 4 * sizeof(uint64_t)) != hipSuccess
         || hipHostMalloc((void**)&p->h_tally, std::max<uint64_t>(1, p->tally_words) * sizeof(uint64_t), 0)
             != hipSuccess

@@ -30,23 +30,37 @@ struct BatchChunk {
 };
 static_assert(sizeof(BatchChunk) == 16, "BatchChunk layout");
 
+/* Host-written control words of a serving kernel (host-coherent memory). */
+struct ServeControl {
+    uint64_t tail; /* chunks published so far */
+    uint64_t stop; /* nonzero: leave once no published chunk is left */
+};
+
 struct BatchArgs {
-    const BatchSet* sets;
-    const BatchChunk* chunks;
-    uint64_t nchunks;
+    const BatchSet* sets; /* batch: device table; serve: host-coherent table */
+    const BatchChunk* chunks; /* batch: device queue; serve: host-coherent ring */
+    uint64_t nchunks; /* batch: queue length */
     uint32_t shells_max; /* LDS slot size in shells */
     uint32_t copies; /* tally copies per wave (1 or 2) */
     uint32_t flush_interval; /* wave steps between slot flushes */
-    uint32_t pad;
+    uint32_t ring_mask; /* serve: ring size - 1 (power of two) */
     unsigned long long* counters; /* [0] chunk cursor [1] interactions [2] truncated */
     unsigned long long* tally; /* tally pool */
-    unsigned long long* done; /* per set: photons completed and flushed */
+    unsigned long long* done; /* per set: [2s] photons completed and flushed, [2s+1] absorptions */
+    const ServeControl* ctl; /* serve only */
+    uint64_t idle_ticks; /* serve: leave after this long without work (100 MHz ticks) */
 };
+
+/* Serving: a set field of a chunk is (slot | generation << 16), so a slot
+ * reused by a later request never matches a wave's stale set. */
+#define TMC_SERVE_SLOT_BITS 16u
 
 size_t batch_lds_bytes(uint32_t block, uint32_t shells_max, uint32_t copies);
 hipError_t launch_batch(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
 hipError_t launch_batch_clear(unsigned long long* tally, uint64_t tally_words, unsigned long long* done,
                               uint32_t nsets, unsigned long long* counters, hipStream_t stream);
+hipError_t launch_serve(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
+hipError_t serve_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max, uint32_t copies, int* wg_per_cu);
 hipError_t batch_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max, uint32_t copies, int* wg_per_cu);
 
 } // namespace tmc

@@ -12,8 +12,19 @@ event counted from photon.c:21-48, over the kernel's average duration
 measured with HIP events on the library's stream), and the reference CPU
 loop timed on this machine's host cores.
 
+Two more legs of the metric follow, each once:
+  sweep   4096 parameter sets (mu_a log-uniform [0.05,10]/cm, mu_s
+          log-uniform [10,200]/cm, shells {51,101,201,401}, width U[10,100] um,
+          photons log-uniform [2^16,2^22], seed = set index) in one persistent
+          launch, beside the same sets run one launch per set;
+  serve   Poisson arrivals of requests (sweep physics, Pareto(1.5) photons in
+          [2^14,2^24]) at a fixed fraction of the sweep's measured capacity;
+          p50/p99 latency, aggregate photons/s, and a sample of results
+          checked bit for bit against the same request run alone.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons N]
                     [--sampler analytic|rejection] [--no-cpu-baseline]
+                    [--no-sweep] [--no-serve] [--serve-load F] [--serve-seconds T]
 """
 import argparse
 import json
@@ -31,6 +42,11 @@ FLOP_PER_INTERACTION = 40.5  # FP32 FLOPs of photon.c:21-48 per absorption event
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "walk_f32_default_2e30_traffic.txt")
 REF_BENCH = os.path.join(ROOT, "oracle", "_ref", "tiny_mc_bench")
+
+
+def progress(msg):
+    """One line per finished leg on stderr (long runs stay visibly alive)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(max_procs=16, photons_per_proc=1 << 18):
@@ -123,6 +139,75 @@ def rocpd_summary(db_path):
     return "\n".join(lines) + "\n"
 
 
+def sweep_leg():
+    """The batched sweep in one launch vs one launch per set (same sets)."""
+    import tiny_mc_amd
+
+    sets = tiny_mc_amd.sweep_sets(n=4096, seed=0)
+    photons = sum(p.photons for p, _ in sets)
+    # warm the multi-set kernel's code object on a small batch
+    tiny_mc_amd.batch_run(tiny_mc_amd.sweep_sets(n=8, seed=1, photon_scale=1e-3))
+    with tiny_mc_amd.BatchPlan(sets, tiny_mc_amd.BatchOptions(use_graph=True)) as plan:
+        t0 = time.perf_counter()
+        _, st = plan.run()
+        batched_s = time.perf_counter() - t0
+    progress(f"sweep batched: {batched_s:.2f} s; now one launch per set")
+    t0 = time.perf_counter()
+    for p, seed in sets:
+        tiny_mc_amd.run_raw(p, seed)
+    per_set_s = time.perf_counter() - t0
+    return {
+        "sets": len(sets),
+        "photons": photons,
+        "interactions": st.interactions,
+        "seconds": batched_s,
+        "photons_per_s": photons / batched_s,
+        "interactions_per_s": st.interactions / batched_s,
+        "one_launch_per_set_seconds": per_set_s,
+        "one_launch_per_set_photons_per_s": photons / per_set_s,
+        "speedup_vs_one_launch_per_set": per_set_s / batched_s,
+        "chunks": st.chunks,
+        "launch": {"grid": st.grid, "block": st.block, "waves_per_simd": st.waves_per_simd,
+                   "copies": st.copies, "chunk_work": st.chunk_work, "hip_graph": True},
+    }
+
+
+def serve_leg(capacity_interactions_per_s, load=0.5, seconds=15.0, check=16):
+    """Open-loop Poisson load at `load` x capacity for about `seconds`."""
+    import numpy as np
+
+    import tiny_mc_amd
+
+    probe = tiny_mc_amd.serve_load.draw_requests(4000, seed=11)
+    mean_work = tiny_mc_amd.serve_load.mean_request_work(probe)
+    rate = load * capacity_interactions_per_s / mean_work
+    n = max(50, int(rate * seconds))
+    reqs = tiny_mc_amd.serve_load.draw_requests(n, seed=12)
+    with tiny_mc_amd.Server() as srv:
+        progress(f"serve: {n} requests at {rate:.4g}/s ({load:.2f} of capacity)")
+        recs, wall = tiny_mc_amd.serve_load.run_open_loop(srv, reqs, rate, progress=progress)
+        stats = srv.stats()
+    lat = np.array([r["stats"].latency_ms for r in recs])
+    photons = sum(p.photons for p, _ in reqs)
+    inter = sum(r["stats"].interactions for r in recs)
+    rng = np.random.default_rng(5)
+    sample = rng.choice(len(reqs), size=min(check, len(reqs)), replace=False)
+    exact = sum(int(np.array_equal(recs[i]["tally"], tiny_mc_amd.run_raw(*reqs[i])[0])) for i in sample)
+    return {
+        "offered_load": load,
+        "offered_requests_per_s": rate,
+        "requests": n,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()),
+        "photons_per_s": photons / wall,
+        "interactions_per_s": inter / wall,
+        "seconds": wall,
+        "kernel_launches": stats["kernel_launches"],
+        "exact_vs_alone": f"{exact}/{len(sample)}",
+    }
+
+
 def read_traffic():
     """HBM bytes per walk launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -146,6 +231,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--rocpd-summary", metavar="DB", help="print a text summary of a rocprofv3 results database")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-load", type=float, default=0.5)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.rocpd_summary:
         return print(rocpd_summary(args.rocpd_summary), end="")
@@ -163,6 +252,7 @@ def main():
     params = tiny_mc_amd.Params(mu_a=2.0, mu_s=20.0, shells=101, microns_per_shell=50.0, photons=args.photons)
     name, cus = tiny_mc_amd.device_info()
 
+    progress(f"device {name} ({cus} CUs); plain walk: {args.warmup} warmup + {args.steps} timed steps")
     for i in range(args.warmup):
         tiny_mc_amd.run_raw(params, 1000 + i, sampler=sampler)
     kernel_ms, interactions = [], 0
@@ -172,6 +262,7 @@ def main():
         kernel_ms.append(st.kernel_ms)
         interactions += st.interactions
     elapsed = time.perf_counter() - t0  # every call is synchronous
+    progress(f"plain walk: {args.steps * args.photons / elapsed:.4g} photons/s")
 
     photons = args.steps * args.photons
     avg_kernel_s = sum(kernel_ms) / len(kernel_ms) * 1e-3
@@ -216,6 +307,16 @@ def main():
         },
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
     }
+    if result["cpu_baseline"]:
+        progress(f"cpu baseline: {result['cpu_baseline']['value']:.4g} photons/s")
+    if not args.no_sweep:
+        result["sweep"] = sweep_leg()
+        progress(f"sweep: {result['sweep']['photons_per_s']:.4g} photons/s, "
+                 f"{result['sweep']['speedup_vs_one_launch_per_set']:.3f}x one launch per set")
+    if not args.no_serve:
+        cap = result["sweep"]["interactions_per_s"] if "sweep" in result else interactions / elapsed
+        result["serve"] = serve_leg(cap, load=args.serve_load, seconds=args.serve_seconds)
+        progress(f"serve: p50 {result['serve']['p50_ms']:.3g} ms, p99 {result['serve']['p99_ms']:.3g} ms")
     print(json.dumps(result))
 
 

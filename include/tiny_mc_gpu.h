@@ -240,6 +240,8 @@ typedef struct tmc_serve_stats {
     uint64_t photons_completed, interactions_completed;
     uint64_t kernel_launches;
     uint64_t chunks_published;
+    uint64_t chunks_claimed; /* ring indices handed to waves (last observed) */
+    uint64_t pending, active; /* requests waiting for admission / admitted, unfinished */
     double elapsed_s;
 } tmc_serve_stats;
 

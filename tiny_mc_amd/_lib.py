@@ -460,6 +460,9 @@ class _ServeStats(ctypes.Structure):
         ("interactions_completed", ctypes.c_uint64),
         ("kernel_launches", ctypes.c_uint64),
         ("chunks_published", ctypes.c_uint64),
+        ("chunks_claimed", ctypes.c_uint64),
+        ("pending", ctypes.c_uint64),
+        ("active", ctypes.c_uint64),
         ("elapsed_s", ctypes.c_double),
     ]
 

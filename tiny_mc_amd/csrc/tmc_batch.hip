@@ -28,7 +28,9 @@ namespace tmc {
 
 /* Moves one wave's private LDS slot into a set's global sums and zeroes it.
  * Only this wave touches the slot, and a wave's LDS operations complete in
- * order, so plain loads see every earlier ds_add of the wave. */
+ * order, so plain loads see every earlier ds_add of the wave.  Serving uses
+ * system-scope atomics, so the sums live where the host's copies read them. */
+template <bool SERVE>
 __device__ __forceinline__ void flush_slot(unsigned long long* slot, uint32_t shells_max, uint32_t copies,
                                            uint32_t shells, unsigned long long* g, uint32_t lane)
 {
@@ -41,10 +43,17 @@ __device__ __forceinline__ void flush_slot(unsigned long long* slot, uint32_t sh
             lw[s] = 0;
             lw[shells_max + s] = 0;
         }
-        if (a)
-            add128(g + 2 * s, a);
-        if (b)
-            add128(g + 2 * shells + 2 * s, b);
+        if (SERVE) {
+            if (a)
+                add128_sys(g + 2 * s, a);
+            if (b)
+                add128_sys(g + 2 * shells + 2 * s, b);
+        } else {
+            if (a)
+                add128(g + 2 * s, a);
+            if (b)
+                add128(g + 2 * shells + 2 * s, b);
+        }
     }
 }
 
@@ -111,13 +120,18 @@ __device__ __forceinline__ void load_set(SetState& st, const BatchArgs& a, uint3
     st.done = a.done + 2 * (uint64_t)index;
 }
 
-/* Next chunk for the wave (wave-uniform).  Batch: the next index of the
- * fixed queue.  Serving: the next published chunk of the ring, claimed by
- * compare-and-swap so no index is reserved before it exists; returns false
- * when the host asked to stop, or after idle_ticks without work, so every
- * wave leaves. */
+enum ClaimResult { CLAIM_GOT, CLAIM_EMPTY, CLAIM_DONE };
+
+/* Tries to give the wave its next chunk (wave-uniform result).
+ * Batch: the next index of the fixed queue, or DONE past its end.
+ * Serving: the wave takes a ticket (the next ring index) once and keeps it
+ * until the host has published that index; meanwhile the call returns EMPTY
+ * without waiting, so the wave's live lanes keep walking.  Polling reads the
+ * host-coherent tail and ring with relaxed system-scope loads (no cache
+ * invalidation).  DONE once the host has set the stop word. */
 template <bool SERVE>
-__device__ __forceinline__ bool next_chunk(const BatchArgs& a, uint32_t lane, BatchChunk& ch)
+__device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t lane, uint64_t& ticket,
+                                                 BatchChunk& ch)
 {
     if (!SERVE) {
         unsigned long long c = 0;
@@ -125,61 +139,52 @@ __device__ __forceinline__ bool next_chunk(const BatchArgs& a, uint32_t lane, Ba
             c = atomicAdd(a.counters, 1ull);
         c = rfl64(c);
         if (c >= a.nchunks)
-            return false;
+            return CLAIM_DONE;
         ch = a.chunks[c];
-        return true;
+        return CLAIM_GOT;
     }
-    uint64_t got = ~0ull, w0 = 0, w1 = 0;
+    uint64_t t = ticket, w0 = 0, w1 = 0;
+    uint32_t res = CLAIM_EMPTY;
     if (lane == 0) {
-        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        for (;;) {
-            const uint64_t tail = __hip_atomic_load(&a.ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-            unsigned long long c = __hip_atomic_load(a.counters, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (c < tail) {
-                /* ring slot c holds chunk c from its publication until the
-                 * cursor is a full wave count past it, so reading it before
-                 * the claim is safe; a lost race discards the read */
-                const uint64_t* slot = (const uint64_t*)(a.chunks + (c & a.ring_mask));
-                w0 = load_sys(slot);
-                w1 = load_sys(slot + 1);
-                if (atomicCAS(a.counters, c, c + 1) == c) {
-                    got = c;
-                    break;
-                }
-                continue;
-            }
-            if (__hip_atomic_load(&a.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
-                break;
-            if (__builtin_amdgcn_s_memrealtime() - t0 > a.idle_ticks)
-                break;
-            __builtin_amdgcn_s_sleep(32);
+        if (t == ~0ull)
+            t = atomicAdd(a.counters, 1ull);
+        if (t < load_sys(&a.ctl->tail)) {
+            const uint64_t* slot = (const uint64_t*)(a.chunks + (t & a.ring_mask));
+            w0 = load_sys(slot);
+            w1 = load_sys(slot + 1);
+            res = CLAIM_GOT;
+        } else if (load_sys(&a.ctl->stop)) {
+            res = CLAIM_DONE;
         }
     }
-    got = rfl64(got);
-    if (got == ~0ull)
-        return false;
-    w0 = rfl64(w0);
-    w1 = rfl64(w1);
-    ch.set = (uint32_t)w0;
-    ch.count = (uint32_t)(w0 >> 32);
-    ch.first = w1;
-    return true;
+    ticket = rfl64(t);
+    res = rfl32(res);
+    if (res == CLAIM_GOT) {
+        ticket = ~0ull;
+        w0 = rfl64(w0);
+        w1 = rfl64(w1);
+        ch.set = (uint32_t)w0;
+        ch.count = (uint32_t)(w0 >> 32);
+        ch.first = w1;
+    }
+    return (ClaimResult)res;
 }
 
 /* Flushes the slot into the set's sums, then publishes the photons completed
- * since the last flush (and their absorptions); when serving, a system-scope
- * release fence in between keeps the set's counter from running ahead of its
- * tallies as the host sees them. */
+ * since the last publication (and their absorptions).  Serving: the sums and
+ * the counters are system-scope atomics, and the wave waits for every sum
+ * update to complete before it adds to the counters, so a host that reads
+ * done == photons then reads final sums. */
 template <bool SERVE>
 __device__ __forceinline__ void flush_and_publish(unsigned long long* slot, const BatchArgs& a, const SetState& st,
                                                   uint32_t lane, uint64_t& completed, uint64_t& absorbed)
 {
-    flush_slot(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
-    /* serving: the host reads counters and tallies while the kernel runs;
-     * a batch's are read after the kernel ends, which orders them anyway */
+    flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
+    if (!completed && !absorbed)
+        return;
     if (SERVE)
-        __threadfence_system();
-    if (lane == 0 && (completed || absorbed)) {
+        wait_vmem();
+    if (lane == 0) {
         if (SERVE) {
             __hip_atomic_fetch_add(st.done, (unsigned long long)completed, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
@@ -214,6 +219,9 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     BatchChunk pend = { 0u, 0u, 0ull };
     uint64_t wave_interactions = 0, wave_truncated = 0, completed = 0, absorbed = 0;
     uint32_t since_flush = 0;
+    uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
+    uint64_t idle_since = 0; /* serving: realtime when the wave ran dry */
+    uint32_t poll_wait = 0; /* serving: steps before re-polling an unpublished ticket */
 
     bool alive = false;
     Photon ph;
@@ -222,16 +230,34 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     for (;;) {
         uint64_t dead = __ballot(!alive);
         if (dead != 0) {
-            if (pool_next == pool_end && !have_pending && !queue_done) {
+            if (SERVE && poll_wait && dead != ~0ull) {
+                --poll_wait; /* live lanes keep walking; poll again later */
+            } else if (pool_next == pool_end && !have_pending && !queue_done) {
                 BatchChunk ch;
-                if (!next_chunk<SERVE>(a, lane, ch)) {
+                const ClaimResult r = try_claim<SERVE>(a, lane, ticket, ch);
+                if (r == CLAIM_GOT) {
+                    idle_since = 0;
+                    if (ch.set == cur_set) {
+                        pool_next = ch.first;
+                        pool_end = ch.first + ch.count;
+                    } else {
+                        pend = ch;
+                        have_pending = true;
+                    }
+                } else if (r == CLAIM_DONE) {
                     queue_done = true;
-                } else if (ch.set == cur_set) {
-                    pool_next = ch.first;
-                    pool_end = ch.first + ch.count;
-                } else {
-                    pend = ch;
-                    have_pending = true;
+                } else if (SERVE && dead != ~0ull) {
+                    poll_wait = 16;
+                } else if (SERVE) {
+                    /* ran dry: publish what was completed, then wait a little */
+                    if (cur_set != 0xFFFFFFFFu)
+                        flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
+                    const uint64_t now = __builtin_amdgcn_s_memrealtime();
+                    if (idle_since == 0)
+                        idle_since = now;
+                    else if (now - idle_since > a.idle_ticks)
+                        queue_done = true;
+                    __builtin_amdgcn_s_sleep(64);
                 }
             }
             if (have_pending && dead == ~0ull) {
@@ -278,13 +304,16 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
         completed += (uint64_t)__popcll(live & __ballot(!alive));
 
         if (++since_flush >= a.flush_interval) {
+            /* overflow guard only: completions are published on switch / idle */
             since_flush = 0;
-            flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
+            flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
         }
     }
     if (cur_set != 0xFFFFFFFFu)
         flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
     if (lane == 0) {
+        if (SERVE)
+            a.lost[blockIdx.x * (blockDim.x >> 6) + wave] = ticket;
         atomicAdd(a.counters + 1, (unsigned long long)wave_interactions);
         if (wave_truncated)
             atomicAdd(a.counters + 2, (unsigned long long)wave_truncated);

@@ -49,6 +49,7 @@ struct BatchArgs {
     unsigned long long* done; /* per set: [2s] photons completed and flushed, [2s+1] absorptions */
     const ServeControl* ctl; /* serve only */
     uint64_t idle_ticks; /* serve: leave after this long without work (100 MHz ticks) */
+    unsigned long long* lost; /* serve: per wave, the ticket it held when it left (~0: none) */
 };
 
 /* Serving: a set field of a chunk is (slot | generation << 16), so a slot

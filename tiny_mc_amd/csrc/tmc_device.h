@@ -28,6 +28,20 @@ __device__ __forceinline__ void add128(unsigned long long* p, unsigned long long
         atomicAdd(p + 1, 1ull);
 }
 
+/* add128 with system-scope atomics: performed where host copies read
+ * (memory), not in this XCD's L2. */
+__device__ __forceinline__ void add128_sys(unsigned long long* p, unsigned long long v)
+{
+    const unsigned long long old = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (old + v < old)
+        __hip_atomic_fetch_add(p + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+/* Waits until every vector memory operation this wave issued has completed
+ * (s_waitcnt vmcnt(0): GFX9 encoding, expcnt and lgkmcnt left at their
+ * maximum). */
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 /* Rank of this lane among the lanes set in `mask` (lanes below it). */
 __device__ __forceinline__ uint32_t mask_rank(uint64_t mask)
 {

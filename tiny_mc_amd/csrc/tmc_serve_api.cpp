@@ -4,9 +4,13 @@
  * Memory the server keeps for its lifetime:
  *   device      tally arena (128-bit sums of every resident request),
  *               per-slot done counters (photons, absorptions), claim cursor;
- *   host        (coherent, read by the kernel over the bus) the request
- *               table (one BatchSet per slot), the chunk ring and the
- *               control words (published tail, stop flag).
+ *   device      also the request table (one BatchSet per slot), the chunk
+ *               ring and the control words (published tail, stop flag),
+ *               written by the server with host-to-device copies on its copy
+ *               stream (descriptors before the tail that publishes them) and
+ *               polled by the kernel with system-scope loads, so the polling
+ *               never crosses the host bus;
+ *   host        pinned staging mirrors of those tables and copy buffers.
  *
  * The server thread loops:
  *   1. admit queued requests in arrival order while a slot, room in the
@@ -106,8 +110,13 @@ struct tmc_server {
     unsigned long long* d_tally = nullptr;
     unsigned long long* d_done = nullptr;
     unsigned long long* d_counters = nullptr;
-    /* host, coherent (read by the kernel) */
-    tmc::ServeControl* h_ctl = nullptr;
+    unsigned long long* d_lost = nullptr; /* per wave: ticket held at exit */
+    tmc::ServeControl* d_ctl = nullptr;
+    tmc::BatchSet* d_sets = nullptr;
+    tmc::BatchChunk* d_ring = nullptr;
+    /* host, pinned staging mirrors of the device tables */
+    uint64_t* h_ctl = nullptr; /* ring of staged control words */
+    uint32_t ctl_next = 0;
     tmc::BatchSet* h_sets = nullptr;
     tmc::BatchChunk* h_ring = nullptr;
     /* host, pinned (copy staging) */
@@ -115,6 +124,7 @@ struct tmc_server {
     unsigned long long* h_zero = nullptr;
     unsigned long long* h_stage = nullptr;
     unsigned long long* h_counters = nullptr;
+    unsigned long long* h_lost = nullptr;
     size_t zero_words = 0;
     tmc::BatchArgs args;
 
@@ -155,8 +165,13 @@ void server_free(tmc_server* s)
         (void)hipFree(s->d_done);
     if (s->d_counters)
         (void)hipFree(s->d_counters);
+    if (s->d_lost)
+        (void)hipFree(s->d_lost);
+    for (void* d : { (void*)s->d_ctl, (void*)s->d_sets, (void*)s->d_ring })
+        if (d)
+            (void)hipFree(d);
     for (void* h : { (void*)s->h_ctl, (void*)s->h_sets, (void*)s->h_ring, (void*)s->h_done, (void*)s->h_zero,
-                     (void*)s->h_stage, (void*)s->h_counters })
+                     (void*)s->h_stage, (void*)s->h_counters, (void*)s->h_lost })
         if (h)
             (void)hipHostFree(h);
     if (s->kstream)
@@ -181,6 +196,70 @@ uint64_t chunk_photons(const tmc_serve_options& o, double L, uint64_t N)
     return std::max<uint64_t>(1, std::min<uint64_t>(N, 64u * (uint64_t)per_lane));
 }
 
+constexpr uint32_t kCtlStage = 1024;
+
+/* Copies a control word (tail or stop) to the device through its own
+ * staging word, so a later update never changes a copy already queued. */
+void put_ctl(tmc_server* s, size_t offset_words, uint64_t v)
+{
+    uint64_t* w = s->h_ctl + (s->ctl_next++ % kCtlStage);
+    *w = v;
+    if (hipMemcpyAsync((uint64_t*)s->d_ctl + offset_words, w, sizeof(uint64_t), hipMemcpyHostToDevice, s->cstream)
+        != hipSuccess)
+        s->error = TMC_E_HIP;
+}
+
+/* Copies ring entries [from, to) from the staging mirror to the device
+ * ring, then the new tail that publishes them (same stream: in order). */
+void publish(tmc_server* s, uint64_t from, uint64_t to)
+{
+    const uint64_t ring = s->opt.ring_chunks;
+    uint64_t p = from;
+    while (p < to) {
+        const uint64_t i = p & (ring - 1);
+        const uint64_t n = std::min(to - p, ring - i);
+        if (hipMemcpyAsync(s->d_ring + i, s->h_ring + i, n * sizeof(tmc::BatchChunk), hipMemcpyHostToDevice,
+                           s->cstream)
+            != hipSuccess)
+            s->error = TMC_E_HIP;
+        p += n;
+    }
+    put_ctl(s, 0, to);
+}
+
+/* After a kernel has left: tickets [0, C) were handed out (C = claim
+ * cursor); every one was walked except those a wave still held when it left
+ * (d_lost).  Chunks published under a lost ticket are published again at the
+ * tail; ring indices handed out beyond the tail are skipped. */
+void recover_tickets(tmc_server* s)
+{
+    const uint64_t ring = s->opt.ring_chunks;
+    if (hipMemcpyAsync(s->h_lost, s->d_lost, s->total_waves * sizeof(uint64_t), hipMemcpyDeviceToHost, s->cstream)
+            != hipSuccess
+        || hipMemcpyAsync(s->h_counters, s->d_counters, sizeof(uint64_t), hipMemcpyDeviceToHost, s->cstream)
+            != hipSuccess
+        || hipStreamSynchronize(s->cstream) != hipSuccess) {
+        s->error = TMC_E_HIP;
+        return;
+    }
+    const uint64_t C = s->h_counters[0];
+    const uint64_t P0 = s->published;
+    uint64_t pub = std::max(P0, C);
+    for (uint32_t w = 0; w < s->total_waves; ++w) {
+        const uint64_t t = s->h_lost[w];
+        if (t == ~0ull || t >= P0)
+            continue;
+        s->h_ring[pub & (ring - 1)] = s->h_ring[t & (ring - 1)];
+        ++pub;
+    }
+    s->claimed = C;
+    if (pub != s->published) {
+        const uint64_t from = std::max(P0, C);
+        s->published = pub;
+        publish(s, from, pub); /* indices handed out beyond the old tail are skipped */
+    }
+}
+
 /* One pass of the server loop; called with s->m held. */
 void serve_step(tmc_server* s)
 {
@@ -199,7 +278,7 @@ void serve_step(tmc_server* s)
         s->slot_gen[r->slot] = r->gen;
         r->tally_off = off;
         r->t_admit = s->now();
-        tmc::BatchSet& d = s->h_sets[r->slot];
+        tmc::BatchSet& d = s->h_sets[r->slot]; /* staging; copied below */
         const tmc_params& q = r->set.params;
         tmc::BatchSet b;
         std::memset(&b, 0, sizeof b);
@@ -212,6 +291,7 @@ void serve_step(tmc_server* s)
         b.shells = q.shells;
         b.kcap = 0xFFFFFFFFu;
         std::memcpy(&d, &b, sizeof b);
+        (void)hipMemcpyAsync(s->d_sets + r->slot, &d, sizeof d, hipMemcpyHostToDevice, s->cstream);
         (void)hipMemcpyAsync(s->d_tally + off, s->h_zero, r->words * sizeof(uint64_t), hipMemcpyHostToDevice,
                              s->cstream);
         (void)hipMemcpyAsync(s->d_done + 2 * (uint64_t)r->slot, s->h_zero, 2 * sizeof(uint64_t),
@@ -219,12 +299,10 @@ void serve_step(tmc_server* s)
         admitted.push_back(r);
         s->st.admitted++;
     }
-    if (!admitted.empty()) {
-        if (hipStreamSynchronize(s->cstream) != hipSuccess)
-            s->error = TMC_E_HIP;
-        for (Request* r : admitted)
-            s->active.push_back(r);
-    }
+    /* the copies above precede, on the copy stream, the tail that publishes
+     * the new requests' first chunks */
+    for (Request* r : admitted)
+        s->active.push_back(r);
 
     /* 2. publish round-robin, bounded backlog */
     const uint64_t backlog_max = std::min<uint64_t>(ring - s->total_waves, 4ull * s->total_waves);
@@ -249,16 +327,18 @@ void serve_step(tmc_server* s)
         s->rr = s->active.empty() ? 0 : (s->rr + 1) % s->active.size();
     }
     if (s->published != before) {
-        std::atomic_thread_fence(std::memory_order_release);
-        __atomic_store_n(&s->h_ctl->tail, s->published, __ATOMIC_RELEASE);
+        publish(s, before, s->published);
         s->st.chunks_published = s->published;
     }
 
     /* 3. keep a kernel running while published work waits */
-    if (s->kernel_running && hipStreamQuery(s->kstream) == hipSuccess)
+    if (s->kernel_running && hipStreamQuery(s->kstream) == hipSuccess) {
         s->kernel_running = false;
-    if (!s->kernel_running && s->published > s->claimed) {
-        if (tmc::launch_serve(s->opt.sampler, s->args, s->grid, s->block, s->kstream) != hipSuccess) {
+        recover_tickets(s);
+    }
+    if (!s->kernel_running && s->published > s->claimed && s->error == TMC_OK) {
+        if (hipMemsetAsync(s->d_lost, 0xFF, s->total_waves * sizeof(uint64_t), s->kstream) != hipSuccess
+            || tmc::launch_serve(s->opt.sampler, s->args, s->grid, s->block, s->kstream) != hipSuccess) {
             s->error = TMC_E_HIP;
         } else {
             s->kernel_running = true;
@@ -269,8 +349,11 @@ void serve_step(tmc_server* s)
     /* 4. completions */
     if (s->active.empty())
         return;
-    if (hipMemcpyAsync(s->h_done, s->d_done, 2 * (size_t)s->opt.max_requests * sizeof(uint64_t),
-                       hipMemcpyDeviceToHost, s->cstream) != hipSuccess
+    uint32_t slots_used = 0;
+    for (Request* r : s->active)
+        slots_used = std::max(slots_used, r->slot + 1);
+    if (hipMemcpyAsync(s->h_done, s->d_done, 2 * (size_t)slots_used * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                       s->cstream) != hipSuccess
         || hipMemcpyAsync(s->h_counters, s->d_counters, sizeof(uint64_t), hipMemcpyDeviceToHost, s->cstream)
             != hipSuccess
         || hipStreamSynchronize(s->cstream) != hipSuccess) {
@@ -323,8 +406,10 @@ void serve_loop(tmc_server* s)
             break;
         if (idle) {
             s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
-            if (s->kernel_running && hipStreamQuery(s->kstream) == hipSuccess)
+            if (s->kernel_running && hipStreamQuery(s->kstream) == hipSuccess) {
                 s->kernel_running = false;
+                recover_tickets(s);
+            }
             continue;
         }
         serve_step(s);
@@ -337,7 +422,8 @@ void serve_loop(tmc_server* s)
         std::this_thread::sleep_for(std::chrono::microseconds(100));
         lk.lock();
     }
-    __atomic_store_n(&s->h_ctl->stop, 1ull, __ATOMIC_RELEASE);
+    put_ctl(s, 1, 1);
+    (void)hipStreamSynchronize(s->cstream);
     lk.unlock();
     (void)hipStreamSynchronize(s->kstream);
 }
@@ -413,10 +499,14 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
         || hipMalloc((void**)&s->d_tally, o.tally_words * sizeof(uint64_t)) != hipSuccess
         || hipMalloc((void**)&s->d_done, 2 * R * sizeof(uint64_t)) != hipSuccess
         || hipMalloc((void**)&s->d_counters, 4 * sizeof(uint64_t)) != hipSuccess
-        || hipHostMalloc((void**)&s->h_ctl, sizeof(tmc::ServeControl), hipHostMallocCoherent) != hipSuccess
-        || hipHostMalloc((void**)&s->h_sets, R * sizeof(tmc::BatchSet), hipHostMallocCoherent) != hipSuccess
-        || hipHostMalloc((void**)&s->h_ring, o.ring_chunks * sizeof(tmc::BatchChunk), hipHostMallocCoherent)
-            != hipSuccess
+        || hipMalloc((void**)&s->d_lost, s->total_waves * sizeof(uint64_t)) != hipSuccess
+        || hipHostMalloc((void**)&s->h_lost, s->total_waves * sizeof(uint64_t), 0) != hipSuccess
+        || hipMalloc((void**)&s->d_ctl, sizeof(tmc::ServeControl)) != hipSuccess
+        || hipMalloc((void**)&s->d_sets, R * sizeof(tmc::BatchSet)) != hipSuccess
+        || hipMalloc((void**)&s->d_ring, o.ring_chunks * sizeof(tmc::BatchChunk)) != hipSuccess
+        || hipHostMalloc((void**)&s->h_ctl, kCtlStage * sizeof(uint64_t), 0) != hipSuccess
+        || hipHostMalloc((void**)&s->h_sets, R * sizeof(tmc::BatchSet), 0) != hipSuccess
+        || hipHostMalloc((void**)&s->h_ring, o.ring_chunks * sizeof(tmc::BatchChunk), 0) != hipSuccess
         || hipHostMalloc((void**)&s->h_done, 2 * R * sizeof(uint64_t), 0) != hipSuccess
         || hipHostMalloc((void**)&s->h_zero, s->zero_words * sizeof(uint64_t), 0) != hipSuccess
         || hipHostMalloc((void**)&s->h_stage, std::max<uint64_t>(o.tally_words, 1) * sizeof(uint64_t), 0)
@@ -425,12 +515,13 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
         server_free(s);
         return TMC_E_NOMEM;
     }
-    std::memset(s->h_ctl, 0, sizeof(tmc::ServeControl));
+    std::memset(s->h_ctl, 0, kCtlStage * sizeof(uint64_t));
     std::memset(s->h_sets, 0, R * sizeof(tmc::BatchSet));
     std::memset(s->h_zero, 0, s->zero_words * sizeof(uint64_t));
     const unsigned long long zeros[4] = { 0, 0, 0, 0 };
     if (hipMemcpy(s->d_counters, zeros, sizeof zeros, hipMemcpyHostToDevice) != hipSuccess
-        || hipMemset(s->d_done, 0, 2 * R * sizeof(uint64_t)) != hipSuccess) {
+        || hipMemset(s->d_done, 0, 2 * R * sizeof(uint64_t)) != hipSuccess
+        || hipMemset(s->d_ctl, 0, sizeof(tmc::ServeControl)) != hipSuccess) {
         server_free(s);
         return TMC_E_HIP;
     }
@@ -440,8 +531,8 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
         s->free_slots.push_back((uint32_t)(R - 1 - i));
 
     std::memset(&s->args, 0, sizeof s->args);
-    s->args.sets = s->h_sets;
-    s->args.chunks = s->h_ring;
+    s->args.sets = s->d_sets;
+    s->args.chunks = s->d_ring;
     s->args.nchunks = 0;
     s->args.shells_max = o.max_shells;
     s->args.copies = 1;
@@ -450,8 +541,9 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
     s->args.counters = s->d_counters;
     s->args.tally = s->d_tally;
     s->args.done = s->d_done;
-    s->args.ctl = s->h_ctl;
+    s->args.ctl = s->d_ctl;
     s->args.idle_ticks = (uint64_t)o.idle_ms * 100000ull; /* 100 MHz realtime counter */
+    s->args.lost = s->d_lost;
 
     s->t0 = Clock::now();
     s->worker = std::thread(serve_loop, s);
@@ -568,6 +660,9 @@ TMC_EXPORT int tmc_serve_get_stats(tmc_server* s, tmc_serve_stats* out)
         return TMC_E_INVALID;
     std::lock_guard<std::mutex> lk(s->m);
     *out = s->st;
+    out->chunks_claimed = s->claimed;
+    out->pending = s->pending.size();
+    out->active = s->active.size();
     out->elapsed_s = s->now();
     return s->error;
 }

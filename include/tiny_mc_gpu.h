@@ -221,9 +221,9 @@ typedef struct tmc_serve_options {
     uint32_t block; /* threads per workgroup; 0 = 256 */
     uint64_t tally_words; /* device tally arena in 64-bit words; 0 = max_requests * 4 * max_shells */
     uint64_t ring_chunks; /* chunk ring entries, power of two; 0 = 2^20 */
-    uint64_t chunk_work; /* target absorptions per chunk; 0 = 2^20 */
+    uint64_t chunk_work; /* target absorptions per chunk; 0 = 2^17 */
     uint32_t max_photons_per_lane; /* cap on photons per lane in a chunk; 0 = 16 */
-    uint32_t idle_ms; /* the kernel leaves after this long without work; 0 = 50 */
+    uint32_t idle_ms; /* the server stops the kernel after this long without requests; 0 = 50 */
 } tmc_serve_options;
 
 typedef struct tmc_request_stats {

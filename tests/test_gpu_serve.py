@@ -87,6 +87,40 @@ def test_kernel_relaunch_after_idle(tmc):
     np.testing.assert_array_equal(b[0], alone(tmc, tmc.Params(photons=50000), 2))
 
 
+def test_burst_past_ring_while_waves_dry(tmc):
+    """Most waves sit dry for longer than idle_ms while one long request
+    runs; then a burst publishes more chunks than the ring holds.  Every
+    ring index a wave took while dry must still be claimed (a wave that left
+    on its own with its ticket stranded that chunk once the ring wrapped)."""
+    long_req = (tmc.Params(mu_a=0.01, mu_s=100.0, shells=101, photons=64), 5)
+    burst = [(tmc.Params(photons=65536, shells=101), 100 + i) for i in range(40)]
+    with tmc.Server(idle_ms=10, ring_chunks=1 << 15, max_photons_per_lane=1, max_shells=101) as srv:
+        first = srv.submit(*long_req)
+        time.sleep(0.05)
+        ids = [srv.submit(p, s) for p, s in burst]
+        got = [srv.wait(i) for i in ids]
+        a = srv.wait(first)
+        st = srv.stats()
+    assert st["chunks_published"] > 1 << 15
+    np.testing.assert_array_equal(a[0], alone(tmc, *long_req))
+    for (p, s), (t, _) in zip(burst, got):
+        np.testing.assert_array_equal(t, alone(tmc, p, s))
+
+
+def test_single_runs_while_serving(tmc):
+    """A single run on the same device while the serving kernel is resident
+    with a long request completes (in the compute units the server leaves
+    free) and matches the same run on an idle device."""
+    long_req = (tmc.Params(mu_a=0.01, mu_s=100.0, shells=101, photons=64), 9)
+    small = (tmc.Params(photons=20000), 3)
+    with tmc.Server(max_shells=101) as srv:
+        rid = srv.submit(*long_req)
+        during = alone(tmc, *small)
+        a = srv.wait(rid)
+    np.testing.assert_array_equal(during, alone(tmc, *small))
+    np.testing.assert_array_equal(a[0], alone(tmc, *long_req))
+
+
 def test_close_finishes_submitted(tmc):
     srv = tmc.Server()
     for i in range(5):

@@ -19,6 +19,20 @@
  * The per-absorption arithmetic is the same step<>() as the single-run
  * kernel, so each set's tallies are bit-identical to a single run of that
  * set (the stream is keyed by (seed, photon index) only).
+ *
+ * Serving (SERVE = true) runs the same loop as a resident kernel fed by a
+ * server thread through host-coherent memory only, so the server never
+ * waits on a copy queued behind some other kernel:
+ *   - the request table, the chunk ring and the control words (tail, stop)
+ *     live in host memory; waves read them with system-scope loads, the
+ *     tail through a device-memory cache that one wave at a time refreshes;
+ *   - a wave takes a ticket (ring index) and keeps it until the tail passes
+ *     it; a wave leaves only when the host has asked this launch to stop,
+ *     which the host does only when every published chunk is done;
+ *   - the wave whose completion count reaches a request's photons moves
+ *     the request's sums (and absorptions) into a host mirror, zeroes them
+ *     on the device for the slot's next request, and then writes the
+ *     request's token into the slot's host flag.
  */
 #include "tmc_batch_internal.h"
 #include "tmc_device.h"
@@ -65,11 +79,28 @@ struct SetState {
     uint64_t first_photon;
     unsigned long long* tally;
     unsigned long long* done;
+    /* serving */
+    uint64_t tally_offset;
+    uint64_t photons;
+    uint64_t token;
+    uint32_t slot;
 };
 
 __device__ __forceinline__ uint64_t load_sys(const uint64_t* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ void store_sys(uint64_t* p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+/* Reads a word at the point where the system-scope atomics are performed
+ * and leaves zero behind. */
+__device__ __forceinline__ unsigned long long take_sys(unsigned long long* p)
+{
+    return __hip_atomic_exchange(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 /* Loads set `index` into the wave-uniform state.  Batch tables are device
@@ -79,45 +110,39 @@ __device__ __forceinline__ uint64_t load_sys(const uint64_t* p)
 template <bool SERVE>
 __device__ __forceinline__ void load_set(SetState& st, const BatchArgs& a, uint32_t index, uint32_t lane)
 {
-    uint64_t w[4];
+    /* words: seed, first_photon, tally_offset, (albedo | spm << 32),
+     * (shell_max | shells << 32), (kcap | pad << 32), photons, token */
+    constexpr int NW = SERVE ? 8 : 6;
+    uint64_t w[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
     const uint64_t* src = (const uint64_t*)(a.sets + index);
     if (SERVE) {
-        uint64_t v[4] = { 0, 0, 0, 0 };
+        uint64_t v[NW];
+        for (int i = 0; i < NW; ++i)
+            v[i] = 0;
         if (lane == 0)
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < NW; ++i)
                 v[i] = load_sys(src + i);
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < NW; ++i)
             w[i] = rfl64(v[i]);
     } else {
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < NW; ++i)
             w[i] = src[i];
-    }
-    /* words: seed, first_photon, tally_offset, (albedo | spm << 32),
-     * then (shell_max | shells << 32), kcap in the next word */
-    uint64_t w4 = 0, w5 = 0;
-    if (SERVE) {
-        uint64_t v4 = 0, v5 = 0;
-        if (lane == 0) {
-            v4 = load_sys(src + 4);
-            v5 = load_sys(src + 5);
-        }
-        w4 = rfl64(v4);
-        w5 = rfl64(v5);
-    } else {
-        w4 = src[4];
-        w5 = src[5];
     }
     st.phys.key0 = (uint32_t)w[0];
     st.phys.key1 = (uint32_t)(w[0] >> 32);
     st.first_photon = w[1];
+    st.tally_offset = w[2];
     st.tally = a.tally + w[2];
     st.phys.albedo = __uint_as_float((uint32_t)w[3]);
     st.phys.spm = __uint_as_float((uint32_t)(w[3] >> 32));
-    st.phys.shell_max = __uint_as_float((uint32_t)w4);
-    st.shells = (uint32_t)(w4 >> 32);
+    st.phys.shell_max = __uint_as_float((uint32_t)w[4]);
+    st.shells = (uint32_t)(w[4] >> 32);
     st.phys.last_shell = st.shells - 1;
-    st.kcap = (uint32_t)w5;
+    st.kcap = (uint32_t)w[5];
     st.done = a.done + 2 * (uint64_t)index;
+    st.photons = w[6];
+    st.token = w[7];
+    st.slot = index;
 }
 
 enum ClaimResult { CLAIM_GOT, CLAIM_EMPTY, CLAIM_DONE };
@@ -125,10 +150,13 @@ enum ClaimResult { CLAIM_GOT, CLAIM_EMPTY, CLAIM_DONE };
 /* Tries to give the wave its next chunk (wave-uniform result).
  * Batch: the next index of the fixed queue, or DONE past its end.
  * Serving: the wave takes a ticket (the next ring index) once and keeps it
- * until the host has published that index; meanwhile the call returns EMPTY
- * without waiting, so the wave's live lanes keep walking.  Polling reads the
- * host-coherent tail and ring with relaxed system-scope loads (no cache
- * invalidation).  DONE once the host has set the stop word. */
+ * until the tail passes it; meanwhile the call returns EMPTY without
+ * waiting, so the wave's live lanes keep walking.  The tail is read from
+ * the device cache; when the ticket lies past it and the cache is older
+ * than the refresh interval, the one wave that wins the refresh reads the
+ * host's words (stop first: a stop seen means the tail read after it is
+ * final).  DONE once this launch is asked to stop and the ticket lies past
+ * the tail. */
 template <bool SERVE>
 __device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t lane, uint64_t& ticket,
                                                  BatchChunk& ch)
@@ -147,14 +175,38 @@ __device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t la
     uint32_t res = CLAIM_EMPTY;
     if (lane == 0) {
         if (t == ~0ull)
-            t = atomicAdd(a.counters, 1ull);
-        if (t < load_sys(&a.ctl->tail)) {
+            t = a.ticket_base + atomicAdd(a.counters, 1ull);
+        uint64_t tail = load_sys(&a.dctl->tail);
+        uint64_t stop = 0;
+        bool stop_read = false;
+        if (t >= tail) {
+            const uint64_t now = __builtin_amdgcn_s_memrealtime();
+            uint64_t last = load_sys(&a.dctl->time);
+            if (now - last >= a.refresh_ticks
+                && __hip_atomic_compare_exchange_strong(&a.dctl->time, &last, now, __ATOMIC_RELAXED,
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
+                stop = load_sys(&a.hctl->stop);
+                wait_vmem();
+                const uint64_t ht = load_sys(&a.hctl->tail);
+                __hip_atomic_fetch_max(&a.dctl->tail, ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_exchange(&a.dctl->stop, stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                tail = ht > tail ? ht : tail;
+                stop_read = true;
+            }
+        }
+        if (t < tail) {
             const uint64_t* slot = (const uint64_t*)(a.chunks + (t & a.ring_mask));
             w0 = load_sys(slot);
             w1 = load_sys(slot + 1);
             res = CLAIM_GOT;
-        } else if (load_sys(&a.ctl->stop)) {
-            res = CLAIM_DONE;
+            /* progress report for the host's backlog bound (a stale
+             * value only makes the host publish less) */
+            store_sys(&a.hctl->claimed, t + 1);
+        } else {
+            if (!stop_read)
+                stop = load_sys(&a.dctl->stop);
+            if (stop == a.launch_id)
+                res = CLAIM_DONE;
         }
     }
     ticket = rfl64(t);
@@ -170,11 +222,33 @@ __device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t la
     return (ClaimResult)res;
 }
 
+/* Serving: called by the wave whose publication completed the request.
+ * Every other contribution to its sums and counters was performed before
+ * the completing count (each wave waits for its sum updates before it
+ * counts), so exchanging each word with zero reads the final sums and
+ * leaves the region clear for the slot's next request.  The sums go to
+ * the host mirror, then, once those writes have completed, the token goes
+ * to the slot's flag. */
+__device__ __forceinline__ void finish_request(const BatchArgs& a, const SetState& st, uint32_t lane)
+{
+    const uint32_t words = 4 * st.shells;
+    for (uint32_t i = lane; i < words; i += 64)
+        store_sys((uint64_t*)a.out + st.tally_offset + i, take_sys(st.tally + i));
+    if (lane == 0) {
+        const unsigned long long absorbed = take_sys(st.done + 1);
+        take_sys(st.done);
+        store_sys((uint64_t*)a.flags + 2 * (uint64_t)st.slot + 1, absorbed);
+    }
+    wait_vmem();
+    if (lane == 0)
+        store_sys((uint64_t*)a.flags + 2 * (uint64_t)st.slot, st.token);
+}
+
 /* Flushes the slot into the set's sums, then publishes the photons completed
  * since the last publication (and their absorptions).  Serving: the sums and
  * the counters are system-scope atomics, and the wave waits for every sum
- * update to complete before it adds to the counters, so a host that reads
- * done == photons then reads final sums. */
+ * update to complete before it adds to the counters, so the wave whose
+ * count completes the request finds final sums. */
 template <bool SERVE>
 __device__ __forceinline__ void flush_and_publish(unsigned long long* slot, const BatchArgs& a, const SetState& st,
                                                   uint32_t lane, uint64_t& completed, uint64_t& absorbed)
@@ -182,18 +256,22 @@ __device__ __forceinline__ void flush_and_publish(unsigned long long* slot, cons
     flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
     if (!completed && !absorbed)
         return;
-    if (SERVE)
+    if (SERVE) {
         wait_vmem();
-    if (lane == 0) {
-        if (SERVE) {
-            __hip_atomic_fetch_add(st.done, (unsigned long long)completed, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_SYSTEM);
+        uint32_t last = 0;
+        if (lane == 0) {
             __hip_atomic_fetch_add(st.done + 1, (unsigned long long)absorbed, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
-        } else {
-            atomicAdd(st.done, (unsigned long long)completed);
-            atomicAdd(st.done + 1, (unsigned long long)absorbed);
+            wait_vmem();
+            const unsigned long long old = __hip_atomic_fetch_add(st.done, (unsigned long long)completed,
+                                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            last = completed && old + completed == st.photons;
         }
+        if (rfl32(last))
+            finish_request(a, st, lane);
+    } else if (lane == 0) {
+        atomicAdd(st.done, (unsigned long long)completed);
+        atomicAdd(st.done + 1, (unsigned long long)absorbed);
     }
     completed = 0;
     absorbed = 0;
@@ -220,7 +298,6 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     uint64_t wave_interactions = 0, wave_truncated = 0, completed = 0, absorbed = 0;
     uint32_t since_flush = 0;
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
-    uint64_t idle_since = 0; /* serving: realtime when the wave ran dry */
     uint32_t poll_wait = 0; /* serving: steps before re-polling an unpublished ticket */
 
     bool alive = false;
@@ -236,7 +313,6 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 BatchChunk ch;
                 const ClaimResult r = try_claim<SERVE>(a, lane, ticket, ch);
                 if (r == CLAIM_GOT) {
-                    idle_since = 0;
                     if (ch.set == cur_set) {
                         pool_next = ch.first;
                         pool_end = ch.first + ch.count;
@@ -249,14 +325,14 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 } else if (SERVE && dead != ~0ull) {
                     poll_wait = 16;
                 } else if (SERVE) {
-                    /* ran dry: publish what was completed, then wait a little */
+                    /* ran dry: publish what was completed, then wait a
+                     * little.  A wave never leaves on its own: the ticket it
+                     * holds would strand the chunk later published there.
+                     * The host asks the launch to stop only once nothing is
+                     * active, when every outstanding ticket lies past the
+                     * tail. */
                     if (cur_set != 0xFFFFFFFFu)
                         flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
-                    const uint64_t now = __builtin_amdgcn_s_memrealtime();
-                    if (idle_since == 0)
-                        idle_since = now;
-                    else if (now - idle_since > a.idle_ticks)
-                        queue_done = true;
                     __builtin_amdgcn_s_sleep(64);
                 }
             }
@@ -312,11 +388,15 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     if (cur_set != 0xFFFFFFFFu)
         flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
     if (lane == 0) {
-        if (SERVE)
-            a.lost[blockIdx.x * (blockDim.x >> 6) + wave] = ticket;
         atomicAdd(a.counters + 1, (unsigned long long)wave_interactions);
         if (wave_truncated)
             atomicAdd(a.counters + 2, (unsigned long long)wave_truncated);
+        /* serving: the last wave to leave rewinds the ticket counter for the
+         * next launch (no wave takes a ticket after it has left) */
+        if (SERVE && atomicAdd(a.counters + 3, 1ull) == a.total_waves - 1) {
+            atomicExch(a.counters, 0ull);
+            atomicExch(a.counters + 3, 0ull);
+        }
     }
 }
 

@@ -18,7 +18,9 @@ struct BatchSet {
     float shell_max; /* (float)(shells - 1) */
     uint32_t shells;
     uint32_t kcap; /* stop a photon after kcap absorptions */
-    uint32_t pad[5];
+    uint32_t pad;
+    uint64_t photons; /* serving: photons of the request */
+    uint64_t token; /* serving: written to the slot's flag when the request is done */
 };
 static_assert(sizeof(BatchSet) == 64, "BatchSet layout");
 
@@ -30,10 +32,22 @@ struct BatchChunk {
 };
 static_assert(sizeof(BatchChunk) == 16, "BatchChunk layout");
 
-/* Host-written control words of a serving kernel (host-coherent memory). */
+/* Serving: control words the host writes in host-coherent memory. */
 struct ServeControl {
-    uint64_t tail; /* chunks published so far */
-    uint64_t stop; /* nonzero: leave once no published chunk is left */
+    uint64_t tail; /* chunks published so far (ring indices below it are valid) */
+    uint64_t stop; /* launch id of the kernel asked to leave */
+    uint64_t claimed; /* written by the kernel: one past the highest ring index claimed */
+    uint64_t pad[5];
+};
+
+/* Serving: device-memory copy of the control words, refreshed from the host
+ * by one wave at a time, at most once per refresh interval, so the polling
+ * of thousands of waves stays off the host bus. */
+struct ServeCache {
+    uint64_t tail;
+    uint64_t stop;
+    uint64_t time; /* realtime of the last refresh */
+    uint64_t pad[5];
 };
 
 struct BatchArgs {
@@ -44,12 +58,20 @@ struct BatchArgs {
     uint32_t copies; /* tally copies per wave (1 or 2) */
     uint32_t flush_interval; /* wave steps between slot flushes */
     uint32_t ring_mask; /* serve: ring size - 1 (power of two) */
-    unsigned long long* counters; /* [0] chunk cursor [1] interactions [2] truncated */
+    unsigned long long* counters; /* [0] chunk cursor (serve: tickets of this launch) [1] interactions
+                                     [2] truncated [3] serve: waves that have left */
     unsigned long long* tally; /* tally pool */
     unsigned long long* done; /* per set: [2s] photons completed and flushed, [2s+1] absorptions */
-    const ServeControl* ctl; /* serve only */
-    uint64_t idle_ticks; /* serve: leave after this long without work (100 MHz ticks) */
-    unsigned long long* lost; /* serve: per wave, the ticket it held when it left (~0: none) */
+    /* serving only */
+    ServeControl* hctl; /* host-coherent control words */
+    ServeCache* dctl; /* device cache of the control words */
+    unsigned long long* out; /* host-coherent mirror of the tally pool: sums of finished requests */
+    unsigned long long* flags; /* host-coherent, per slot: [2s] token of the finished request, [2s+1]
+                                  its absorptions */
+    uint64_t ticket_base; /* ring index of this launch's first ticket */
+    uint64_t launch_id; /* the stop word that makes this launch leave */
+    uint32_t total_waves; /* waves of the launch */
+    uint32_t refresh_ticks; /* realtime ticks (100 MHz) between reads of the host control words */
 };
 
 /* Serving: a set field of a chunk is (slot | generation << 16), so a slot
@@ -57,6 +79,7 @@ struct BatchArgs {
 #define TMC_SERVE_SLOT_BITS 16u
 
 size_t batch_lds_bytes(uint32_t block, uint32_t shells_max, uint32_t copies);
+
 hipError_t launch_batch(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
 hipError_t launch_batch_clear(unsigned long long* tally, uint64_t tally_words, unsigned long long* done,
                               uint32_t nsets, unsigned long long* counters, hipStream_t stream);

@@ -2,35 +2,41 @@
  * persistent kernel (include/tiny_mc_gpu.h, tmc_serve_*).
  *
  * Memory the server keeps for its lifetime:
- *   device      tally arena (128-bit sums of every resident request),
- *               per-slot done counters (photons, absorptions), claim cursor;
- *   device      also the request table (one BatchSet per slot), the chunk
- *               ring and the control words (published tail, stop flag),
- *               written by the server with host-to-device copies on its copy
- *               stream (descriptors before the tail that publishes them) and
- *               polled by the kernel with system-scope loads, so the polling
- *               never crosses the host bus;
- *   host        pinned staging mirrors of those tables and copy buffers.
+ *   device          tally arena (128-bit sums of every resident request),
+ *                   per-slot counters (photons completed, absorptions), the
+ *                   ticket counter and a cache of the control words;
+ *   host-coherent   the request table (one BatchSet per slot), the chunk
+ *                   ring, the control words (tail, stop), a mirror of the
+ *                   tally arena that receives finished sums, and per-slot
+ *                   flags.
+ * The server thread makes no copies and waits on no queue while the kernel
+ * runs: it writes descriptors, chunks and the tail with ordinary stores and
+ * reads finished requests from the mirror once their flag holds their
+ * token.  (A copy could sit behind another process's or thread's kernel
+ * that in turn waits for the compute units the resident kernel holds.)
  *
  * The server thread loops:
- *   1. admit queued requests in arrival order while a slot, room in the
- *      tally arena and room in the ring are free (zeroing the request's
- *      tally and counters with copies on the copy stream);
- *   2. publish chunks round-robin over the admitted requests, keeping a
- *      bounded number of published-but-unclaimed chunks, so a request that
+ *   1. notice a kernel that has left (it leaves only when asked to stop);
+ *   2. admit queued requests in arrival order while a slot and room in the
+ *      tally arena are free;
+ *   3. publish chunks, shortest remaining work first, keeping about one
+ *      published-but-unclaimed chunk per resident wave, so a request that
  *      arrives while others run gets its first chunks claimed within about
  *      one chunk time of a wave (this is what folds it into the running
  *      kernel);
- *   3. (re)launch the kernel when published work is waiting and no kernel
- *      is running (the kernel leaves after idle_ms without work);
- *   4. copy the done counters back, and for every request whose photons are
- *      all counted copy its tally back, record its latency, free its slot.
+ *   4. launch the kernel when published work waits and none is running;
+ *   5. finish every request whose flag holds its token: take its sums from
+ *      the mirror, record its latency, free its slot and arena region.
+ * After idle_ms with no request in the server it asks the kernel to stop:
+ * then every published chunk has been claimed and done, so each wave's
+ * ticket lies past the tail and the next launch starts its tickets at the
+ * tail.
  */
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -99,33 +105,32 @@ struct Arena {
     }
 };
 
+/* Workgroup slots the serving kernel leaves free, so other work on the
+ * device (another thread's run, the runtime's copies) still gets compute
+ * units while requests keep the kernel resident. */
+constexpr uint32_t kFreeSlots = 8;
+/* Minimum time between two reads of the host's control words by the
+ * kernel, in 100 MHz realtime ticks (2 us). */
+constexpr uint32_t kRefreshTicks = 200;
+
 } // namespace
 
 struct tmc_server {
     tmc_serve_options opt;
     int device = 0;
     uint32_t grid = 0, block = 0, total_waves = 0;
-    hipStream_t kstream = nullptr, cstream = nullptr;
+    hipStream_t kstream = nullptr;
     /* device */
     unsigned long long* d_tally = nullptr;
     unsigned long long* d_done = nullptr;
     unsigned long long* d_counters = nullptr;
-    unsigned long long* d_lost = nullptr; /* per wave: ticket held at exit */
-    tmc::ServeControl* d_ctl = nullptr;
-    tmc::BatchSet* d_sets = nullptr;
-    tmc::BatchChunk* d_ring = nullptr;
-    /* host, pinned staging mirrors of the device tables */
-    uint64_t* h_ctl = nullptr; /* ring of staged control words */
-    uint32_t ctl_next = 0;
+    tmc::ServeCache* d_cache = nullptr;
+    /* host-coherent */
+    tmc::ServeControl* h_ctl = nullptr;
     tmc::BatchSet* h_sets = nullptr;
     tmc::BatchChunk* h_ring = nullptr;
-    /* host, pinned (copy staging) */
-    unsigned long long* h_done = nullptr;
-    unsigned long long* h_zero = nullptr;
-    unsigned long long* h_stage = nullptr;
-    unsigned long long* h_counters = nullptr;
-    unsigned long long* h_lost = nullptr;
-    size_t zero_words = 0;
+    unsigned long long* h_out = nullptr;
+    unsigned long long* h_flags = nullptr;
     tmc::BatchArgs args;
 
     std::mutex m;
@@ -143,9 +148,13 @@ struct tmc_server {
     std::vector<uint32_t> free_slots;
     std::vector<uint32_t> slot_gen;
     Arena arena;
-    size_t rr = 0;
-    uint64_t published = 0, claimed = 0;
+    uint64_t published = 0; /* ring indices written (the tail) */
+    uint64_t claimed = 0; /* highest claim the kernel reported */
+    uint64_t base = 0; /* first ticket of the next launch */
+    uint64_t launch_id = 0;
     bool kernel_running = false;
+    bool stopping = false; /* the running kernel was asked to stop */
+    double last_busy = 0; /* last time the server held a request */
     tmc_serve_stats st;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -157,27 +166,14 @@ void server_free(tmc_server* s)
 {
     if (s->kstream)
         (void)hipStreamSynchronize(s->kstream);
-    if (s->cstream)
-        (void)hipStreamSynchronize(s->cstream);
-    if (s->d_tally)
-        (void)hipFree(s->d_tally);
-    if (s->d_done)
-        (void)hipFree(s->d_done);
-    if (s->d_counters)
-        (void)hipFree(s->d_counters);
-    if (s->d_lost)
-        (void)hipFree(s->d_lost);
-    for (void* d : { (void*)s->d_ctl, (void*)s->d_sets, (void*)s->d_ring })
+    for (void* d : { (void*)s->d_tally, (void*)s->d_done, (void*)s->d_counters, (void*)s->d_cache })
         if (d)
             (void)hipFree(d);
-    for (void* h : { (void*)s->h_ctl, (void*)s->h_sets, (void*)s->h_ring, (void*)s->h_done, (void*)s->h_zero,
-                     (void*)s->h_stage, (void*)s->h_counters, (void*)s->h_lost })
+    for (void* h : { (void*)s->h_ctl, (void*)s->h_sets, (void*)s->h_ring, (void*)s->h_out, (void*)s->h_flags })
         if (h)
             (void)hipHostFree(h);
     if (s->kstream)
         (void)hipStreamDestroy(s->kstream);
-    if (s->cstream)
-        (void)hipStreamDestroy(s->cstream);
     for (Request* r : s->pending)
         delete r;
     for (Request* r : s->active)
@@ -187,85 +183,61 @@ void server_free(tmc_server* s)
     delete s;
 }
 
-/* Chunk size of a request: about chunk_work absorptions, between 1 and
- * max_photons_per_lane photons per lane, never more than the request. */
+/* Chunk size of a request: about chunk_work absorptions (so a chunk lasts
+ * about chunk_work / 64 steps of a wave, which bounds how long a new
+ * request waits for waves), between 1 and max_photons_per_lane photons per
+ * lane, never more than the request. */
 uint64_t chunk_photons(const tmc_serve_options& o, double L, uint64_t N)
 {
-    double per_lane = std::ceil((double)o.chunk_work / (64.0 * L));
+    double per_lane = std::floor((double)o.chunk_work / (64.0 * L));
     per_lane = std::min(std::max(per_lane, 1.0), (double)o.max_photons_per_lane);
     return std::max<uint64_t>(1, std::min<uint64_t>(N, 64u * (uint64_t)per_lane));
 }
 
-constexpr uint32_t kCtlStage = 1024;
+void store_release(uint64_t* p, uint64_t v) { __atomic_store_n(p, v, __ATOMIC_RELEASE); }
+uint64_t load_acquire(const uint64_t* p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
 
-/* Copies a control word (tail or stop) to the device through its own
- * staging word, so a later update never changes a copy already queued. */
-void put_ctl(tmc_server* s, size_t offset_words, uint64_t v)
+/* Asks the running kernel to stop; called only when no request is admitted
+ * or queued and nothing will be published until it has left. */
+void request_stop(tmc_server* s)
 {
-    uint64_t* w = s->h_ctl + (s->ctl_next++ % kCtlStage);
-    *w = v;
-    if (hipMemcpyAsync((uint64_t*)s->d_ctl + offset_words, w, sizeof(uint64_t), hipMemcpyHostToDevice, s->cstream)
-        != hipSuccess)
-        s->error = TMC_E_HIP;
-}
-
-/* Copies ring entries [from, to) from the staging mirror to the device
- * ring, then the new tail that publishes them (same stream: in order). */
-void publish(tmc_server* s, uint64_t from, uint64_t to)
-{
-    const uint64_t ring = s->opt.ring_chunks;
-    uint64_t p = from;
-    while (p < to) {
-        const uint64_t i = p & (ring - 1);
-        const uint64_t n = std::min(to - p, ring - i);
-        if (hipMemcpyAsync(s->d_ring + i, s->h_ring + i, n * sizeof(tmc::BatchChunk), hipMemcpyHostToDevice,
-                           s->cstream)
-            != hipSuccess)
-            s->error = TMC_E_HIP;
-        p += n;
+    if (s->kernel_running && !s->stopping) {
+        store_release(&s->h_ctl->stop, s->launch_id);
+        s->stopping = true;
     }
-    put_ctl(s, 0, to);
 }
 
-/* After a kernel has left: tickets [0, C) were handed out (C = claim
- * cursor); every one was walked except those a wave still held when it left
- * (d_lost).  Chunks published under a lost ticket are published again at the
- * tail; ring indices handed out beyond the tail are skipped. */
-void recover_tickets(tmc_server* s)
+/* Notices a kernel that has left.  It leaves only after a stop request,
+ * when every published chunk had been claimed, so the next launch starts
+ * its tickets at the tail. */
+void reap_kernel(tmc_server* s)
 {
-    const uint64_t ring = s->opt.ring_chunks;
-    if (hipMemcpyAsync(s->h_lost, s->d_lost, s->total_waves * sizeof(uint64_t), hipMemcpyDeviceToHost, s->cstream)
-            != hipSuccess
-        || hipMemcpyAsync(s->h_counters, s->d_counters, sizeof(uint64_t), hipMemcpyDeviceToHost, s->cstream)
-            != hipSuccess
-        || hipStreamSynchronize(s->cstream) != hipSuccess) {
+    if (!s->kernel_running)
+        return;
+    const hipError_t q = hipStreamQuery(s->kstream);
+    if (q == hipErrorNotReady)
+        return;
+    s->kernel_running = false;
+    s->stopping = false;
+    if (q != hipSuccess) {
         s->error = TMC_E_HIP;
         return;
     }
-    const uint64_t C = s->h_counters[0];
-    const uint64_t P0 = s->published;
-    uint64_t pub = std::max(P0, C);
-    for (uint32_t w = 0; w < s->total_waves; ++w) {
-        const uint64_t t = s->h_lost[w];
-        if (t == ~0ull || t >= P0)
-            continue;
-        s->h_ring[pub & (ring - 1)] = s->h_ring[t & (ring - 1)];
-        ++pub;
-    }
-    s->claimed = C;
-    if (pub != s->published) {
-        const uint64_t from = std::max(P0, C);
-        s->published = pub;
-        publish(s, from, pub); /* indices handed out beyond the old tail are skipped */
-    }
+    s->base = s->published;
+    s->claimed = std::max(s->claimed, s->published);
 }
 
 /* One pass of the server loop; called with s->m held. */
 void serve_step(tmc_server* s)
 {
     const uint64_t ring = s->opt.ring_chunks;
-    /* 1. admit in arrival order */
-    std::vector<Request*> admitted;
+    reap_kernel(s);
+    if (s->error != TMC_OK)
+        return;
+
+    /* admit in arrival order.  Tally regions and counters are zero here:
+     * zeroed at open, and again by the wave that finished the region's
+     * previous request. */
     while (!s->pending.empty() && !s->free_slots.empty()) {
         Request* r = s->pending.front();
         uint64_t off = 0;
@@ -278,7 +250,6 @@ void serve_step(tmc_server* s)
         s->slot_gen[r->slot] = r->gen;
         r->tally_off = off;
         r->t_admit = s->now();
-        tmc::BatchSet& d = s->h_sets[r->slot]; /* staging; copied below */
         const tmc_params& q = r->set.params;
         tmc::BatchSet b;
         std::memset(&b, 0, sizeof b);
@@ -290,110 +261,90 @@ void serve_step(tmc_server* s)
         b.shell_max = (float)(q.shells - 1);
         b.shells = q.shells;
         b.kcap = 0xFFFFFFFFu;
-        std::memcpy(&d, &b, sizeof b);
-        (void)hipMemcpyAsync(s->d_sets + r->slot, &d, sizeof d, hipMemcpyHostToDevice, s->cstream);
-        (void)hipMemcpyAsync(s->d_tally + off, s->h_zero, r->words * sizeof(uint64_t), hipMemcpyHostToDevice,
-                             s->cstream);
-        (void)hipMemcpyAsync(s->d_done + 2 * (uint64_t)r->slot, s->h_zero, 2 * sizeof(uint64_t),
-                             hipMemcpyHostToDevice, s->cstream);
-        admitted.push_back(r);
+        b.photons = q.photons;
+        b.token = r->id;
+        /* read by the kernel only after a chunk of this request is
+         * published, which the release store of the tail orders after */
+        std::memcpy(&s->h_sets[r->slot], &b, sizeof b);
+        s->active.push_back(r);
         s->st.admitted++;
     }
-    /* the copies above precede, on the copy stream, the tail that publishes
-     * the new requests' first chunks */
-    for (Request* r : admitted)
-        s->active.push_back(r);
 
-    /* 2. publish round-robin, bounded backlog */
-    const uint64_t backlog_max = std::min<uint64_t>(ring - s->total_waves, 4ull * s->total_waves);
-    bool any = true;
+    /* publish, shortest remaining work first, keeping about one unclaimed
+     * chunk per resident wave.  A ring index is rewritten only when it is
+     * more than ring - 2 * waves below the claims reported, so every wave
+     * has read it (at most one ticket per wave is outstanding). */
+    s->claimed = std::max<uint64_t>(s->claimed, load_acquire(&s->h_ctl->claimed));
+    const uint64_t backlog_max = std::min<uint64_t>(ring - 2 * s->total_waves, s->total_waves);
+    auto unclaimed = [s]() -> uint64_t { return s->published > s->claimed ? s->published - s->claimed : 0; };
     const uint64_t before = s->published;
-    while (any && s->published - s->claimed < backlog_max && !s->active.empty()) {
-        any = false;
-        for (size_t n = 0; n < s->active.size() && s->published - s->claimed < backlog_max; ++n) {
-            Request* r = s->active[(s->rr + n) % s->active.size()];
+    if (!(s->kernel_running && s->stopping) && unclaimed() < backlog_max) {
+        std::vector<Request*> order;
+        for (Request* r : s->active)
+            if (r->next < r->set.params.photons)
+                order.push_back(r);
+        std::sort(order.begin(), order.end(), [](const Request* x, const Request* y) {
+            return (double)(x->set.params.photons - x->next) * x->L < (double)(y->set.params.photons - y->next) * y->L;
+        });
+        for (Request* r : order) {
             const uint64_t N = r->set.params.photons;
-            if (r->next >= N)
-                continue;
-            const uint64_t cnt = std::min(r->chunk, N - r->next);
-            tmc::BatchChunk& c = s->h_ring[s->published & (ring - 1)];
-            c.set = r->slot | (r->gen << TMC_SERVE_SLOT_BITS);
-            c.count = (uint32_t)cnt;
-            c.first = r->next;
-            r->next += cnt;
-            s->published++;
-            any = true;
+            while (r->next < N && unclaimed() < backlog_max) {
+                const uint64_t cnt = std::min(r->chunk, N - r->next);
+                tmc::BatchChunk& c = s->h_ring[s->published & (ring - 1)];
+                c.set = r->slot | (r->gen << TMC_SERVE_SLOT_BITS);
+                c.count = (uint32_t)cnt;
+                c.first = r->next;
+                r->next += cnt;
+                s->published++;
+            }
+            if (unclaimed() >= backlog_max)
+                break;
         }
-        s->rr = s->active.empty() ? 0 : (s->rr + 1) % s->active.size();
     }
     if (s->published != before) {
-        publish(s, before, s->published);
+        store_release(&s->h_ctl->tail, s->published);
         s->st.chunks_published = s->published;
     }
 
-    /* 3. keep a kernel running while published work waits */
-    if (s->kernel_running && hipStreamQuery(s->kstream) == hipSuccess) {
-        s->kernel_running = false;
-        recover_tickets(s);
-    }
-    if (!s->kernel_running && s->published > s->claimed && s->error == TMC_OK) {
-        if (hipMemsetAsync(s->d_lost, 0xFF, s->total_waves * sizeof(uint64_t), s->kstream) != hipSuccess
-            || tmc::launch_serve(s->opt.sampler, s->args, s->grid, s->block, s->kstream) != hipSuccess) {
+    /* keep a kernel running while published work waits */
+    if (!s->kernel_running && s->published > s->base) {
+        s->args.ticket_base = s->base;
+        s->args.launch_id = ++s->launch_id;
+        if (tmc::launch_serve(s->opt.sampler, s->args, s->grid, s->block, s->kstream) != hipSuccess) {
             s->error = TMC_E_HIP;
-        } else {
-            s->kernel_running = true;
-            s->st.kernel_launches++;
+            return;
         }
+        s->kernel_running = true;
+        s->st.kernel_launches++;
     }
 
-    /* 4. completions */
-    if (s->active.empty())
-        return;
-    uint32_t slots_used = 0;
-    for (Request* r : s->active)
-        slots_used = std::max(slots_used, r->slot + 1);
-    if (hipMemcpyAsync(s->h_done, s->d_done, 2 * (size_t)slots_used * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                       s->cstream) != hipSuccess
-        || hipMemcpyAsync(s->h_counters, s->d_counters, sizeof(uint64_t), hipMemcpyDeviceToHost, s->cstream)
-            != hipSuccess
-        || hipStreamSynchronize(s->cstream) != hipSuccess) {
-        s->error = TMC_E_HIP;
-        return;
-    }
-    s->claimed = std::max<uint64_t>(s->claimed, s->h_counters[0]);
-    std::vector<Request*> done;
-    for (Request* r : s->active)
-        if (r->next >= r->set.params.photons && s->h_done[2 * (uint64_t)r->slot] >= r->set.params.photons)
-            done.push_back(r);
-    if (done.empty())
-        return;
-    uint64_t stage = 0;
-    for (Request* r : done) {
-        (void)hipMemcpyAsync(s->h_stage + stage, s->d_tally + r->tally_off, r->words * sizeof(uint64_t),
-                             hipMemcpyDeviceToHost, s->cstream);
-        stage += r->words;
-    }
-    if (hipStreamSynchronize(s->cstream) != hipSuccess) {
-        s->error = TMC_E_HIP;
-        return;
-    }
-    stage = 0;
+    /* finished requests: the flag holds the token once the sums are in the
+     * mirror */
     const double t = s->now();
-    for (Request* r : done) {
-        r->result.assign(s->h_stage + stage, s->h_stage + stage + r->words);
-        stage += r->words;
-        r->interactions = s->h_done[2 * (uint64_t)r->slot + 1];
+    bool any = false;
+    for (size_t i = 0; i < s->active.size();) {
+        Request* r = s->active[i];
+        if (r->next < r->set.params.photons || load_acquire((const uint64_t*)s->h_flags + 2 * r->slot) != r->id) {
+            ++i;
+            continue;
+        }
+        const unsigned long long* src = s->h_out + r->tally_off;
+        r->result.assign(src, src + r->words);
+        r->interactions = s->h_flags[2 * (uint64_t)r->slot + 1];
         r->t_done = t;
         s->arena.release(r->tally_off, r->words);
         s->free_slots.push_back(r->slot);
-        s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+        s->active[i] = s->active.back();
+        s->active.pop_back();
         s->finished[r->id] = r;
         s->state[r->id] = 1;
         s->st.completed++;
         s->st.photons_completed += r->set.params.photons;
         s->st.interactions_completed += r->interactions;
+        any = true;
     }
-    s->cv_done.notify_all();
+    if (any)
+        s->cv_done.notify_all();
 }
 
 void serve_loop(tmc_server* s)
@@ -406,26 +357,47 @@ void serve_loop(tmc_server* s)
             break;
         if (idle) {
             s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
-            if (s->kernel_running && hipStreamQuery(s->kstream) == hipSuccess) {
-                s->kernel_running = false;
-                recover_tickets(s);
-            }
+            if (s->now() - s->last_busy > 1e-3 * s->opt.idle_ms)
+                request_stop(s);
+            reap_kernel(s);
+            if (s->error != TMC_OK)
+                break;
             continue;
         }
+        s->last_busy = s->now();
         serve_step(s);
-        if (s->error != TMC_OK) {
-            s->cv_done.notify_all();
+        if (s->error != TMC_OK)
             break;
-        }
         /* short pause: other threads submit and poll under the lock */
         lk.unlock();
-        std::this_thread::sleep_for(std::chrono::microseconds(100));
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
         lk.lock();
     }
-    put_ctl(s, 1, 1);
-    (void)hipStreamSynchronize(s->cstream);
-    lk.unlock();
-    (void)hipStreamSynchronize(s->kstream);
+    s->cv_done.notify_all();
+    /* every request is finished (or the server failed): let the kernel go */
+    if (s->kernel_running) {
+        store_release(&s->h_ctl->stop, s->launch_id);
+        lk.unlock();
+        (void)hipStreamSynchronize(s->kstream);
+        lk.lock();
+        s->kernel_running = false;
+    }
+}
+
+template <typename T>
+bool host_alloc(T** p, size_t count)
+{
+    if (hipHostMalloc((void**)p, count * sizeof(T), hipHostMallocCoherent) != hipSuccess)
+        return false;
+    std::memset((void*)*p, 0, count * sizeof(T));
+    return true;
+}
+
+template <typename T>
+T* device_view(T* host)
+{
+    void* d = nullptr;
+    return hipHostGetDevicePointer(&d, host, 0) == hipSuccess ? (T*)d : nullptr;
 }
 
 } // namespace
@@ -457,7 +429,7 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
     if (!o.ring_chunks)
         o.ring_chunks = 1ull << 20;
     if (!o.chunk_work)
-        o.chunk_work = 1ull << 20;
+        o.chunk_work = 1ull << 17;
     if (!o.max_photons_per_lane)
         o.max_photons_per_lane = 16;
     if (!o.idle_ms)
@@ -487,41 +459,27 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
     }
     s->block = o.block;
     s->grid = (uint32_t)wg * (uint32_t)c->compute_units;
+    s->grid = s->grid > 2 * kFreeSlots ? s->grid - kFreeSlots : (s->grid + 1) / 2;
     s->total_waves = s->grid * (o.block / 64);
     if (o.ring_chunks <= 2ull * s->total_waves) {
         delete s;
         return TMC_E_INVALID;
     }
-    s->zero_words = std::max<size_t>(4 * (size_t)o.max_shells, 2);
     const size_t R = o.max_requests;
     if (hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking) != hipSuccess
-        || hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking) != hipSuccess
         || hipMalloc((void**)&s->d_tally, o.tally_words * sizeof(uint64_t)) != hipSuccess
         || hipMalloc((void**)&s->d_done, 2 * R * sizeof(uint64_t)) != hipSuccess
         || hipMalloc((void**)&s->d_counters, 4 * sizeof(uint64_t)) != hipSuccess
-        || hipMalloc((void**)&s->d_lost, s->total_waves * sizeof(uint64_t)) != hipSuccess
-        || hipHostMalloc((void**)&s->h_lost, s->total_waves * sizeof(uint64_t), 0) != hipSuccess
-        || hipMalloc((void**)&s->d_ctl, sizeof(tmc::ServeControl)) != hipSuccess
-        || hipMalloc((void**)&s->d_sets, R * sizeof(tmc::BatchSet)) != hipSuccess
-        || hipMalloc((void**)&s->d_ring, o.ring_chunks * sizeof(tmc::BatchChunk)) != hipSuccess
-        || hipHostMalloc((void**)&s->h_ctl, kCtlStage * sizeof(uint64_t), 0) != hipSuccess
-        || hipHostMalloc((void**)&s->h_sets, R * sizeof(tmc::BatchSet), 0) != hipSuccess
-        || hipHostMalloc((void**)&s->h_ring, o.ring_chunks * sizeof(tmc::BatchChunk), 0) != hipSuccess
-        || hipHostMalloc((void**)&s->h_done, 2 * R * sizeof(uint64_t), 0) != hipSuccess
-        || hipHostMalloc((void**)&s->h_zero, s->zero_words * sizeof(uint64_t), 0) != hipSuccess
-        || hipHostMalloc((void**)&s->h_stage, std::max<uint64_t>(o.tally_words, 1) * sizeof(uint64_t), 0)
-            != hipSuccess
-        || hipHostMalloc((void**)&s->h_counters, 4 * sizeof(uint64_t), 0) != hipSuccess) {
+        || hipMalloc((void**)&s->d_cache, sizeof(tmc::ServeCache)) != hipSuccess
+        || !host_alloc(&s->h_ctl, 1) || !host_alloc(&s->h_sets, R) || !host_alloc(&s->h_ring, o.ring_chunks)
+        || !host_alloc(&s->h_out, o.tally_words) || !host_alloc(&s->h_flags, 2 * R)) {
         server_free(s);
         return TMC_E_NOMEM;
     }
-    std::memset(s->h_ctl, 0, kCtlStage * sizeof(uint64_t));
-    std::memset(s->h_sets, 0, R * sizeof(tmc::BatchSet));
-    std::memset(s->h_zero, 0, s->zero_words * sizeof(uint64_t));
-    const unsigned long long zeros[4] = { 0, 0, 0, 0 };
-    if (hipMemcpy(s->d_counters, zeros, sizeof zeros, hipMemcpyHostToDevice) != hipSuccess
+    if (hipMemset(s->d_tally, 0, o.tally_words * sizeof(uint64_t)) != hipSuccess
         || hipMemset(s->d_done, 0, 2 * R * sizeof(uint64_t)) != hipSuccess
-        || hipMemset(s->d_ctl, 0, sizeof(tmc::ServeControl)) != hipSuccess) {
+        || hipMemset(s->d_counters, 0, 4 * sizeof(uint64_t)) != hipSuccess
+        || hipMemset(s->d_cache, 0, sizeof(tmc::ServeCache)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         server_free(s);
         return TMC_E_HIP;
     }
@@ -531,8 +489,15 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
         s->free_slots.push_back((uint32_t)(R - 1 - i));
 
     std::memset(&s->args, 0, sizeof s->args);
-    s->args.sets = s->d_sets;
-    s->args.chunks = s->d_ring;
+    s->args.sets = device_view(s->h_sets);
+    s->args.chunks = device_view(s->h_ring);
+    s->args.hctl = device_view(s->h_ctl);
+    s->args.out = device_view(s->h_out);
+    s->args.flags = device_view(s->h_flags);
+    if (!s->args.sets || !s->args.chunks || !s->args.hctl || !s->args.out || !s->args.flags) {
+        server_free(s);
+        return TMC_E_HIP;
+    }
     s->args.nchunks = 0;
     s->args.shells_max = o.max_shells;
     s->args.copies = 1;
@@ -541,9 +506,9 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
     s->args.counters = s->d_counters;
     s->args.tally = s->d_tally;
     s->args.done = s->d_done;
-    s->args.ctl = s->d_ctl;
-    s->args.idle_ticks = (uint64_t)o.idle_ms * 100000ull; /* 100 MHz realtime counter */
-    s->args.lost = s->d_lost;
+    s->args.dctl = s->d_cache;
+    s->args.total_waves = s->total_waves;
+    s->args.refresh_ticks = kRefreshTicks;
 
     s->t0 = Clock::now();
     s->worker = std::thread(serve_loop, s);

@@ -61,6 +61,14 @@ typedef struct tmc_params {
 #define TMC_SAMPLER_ANALYTIC 0u /* cos(theta) = 2u-1, phi = 2 pi u' (default) */
 #define TMC_SAMPLER_REJECTION 1u /* the reference's rejection method */
 
+/* Arithmetic of the walk (the tallies are the same integers in every mode).
+ * The reference computes in binary32 (photon.c); the other modes exist to
+ * measure what that costs where it matters most (albedo near 1). */
+#define TMC_PRECISION_FP32 0u /* the reference's binary32 arithmetic (default) */
+#define TMC_PRECISION_FP32_WEIGHT 1u /* binary32, weight from exact counts and FP64 constants */
+#define TMC_PRECISION_FP32_COMPENSATED 2u /* as _WEIGHT, plus compensated position sums */
+#define TMC_PRECISION_FP64 3u /* every quantity in binary64 */
+
 typedef struct tmc_options {
     uint32_t sampler; /* TMC_SAMPLER_* */
     uint32_t block; /* threads per workgroup, multiple of 64 in [64, 1024]; 0 = auto */
@@ -68,6 +76,8 @@ typedef struct tmc_options {
     uint32_t chunk; /* photons a wave claims at a time, multiple of 64; 0 = auto */
     uint64_t first_photon; /* index of the first photon of the stream to walk */
     uint64_t max_interactions; /* stop a photon after this many absorptions; 0 = never */
+    uint32_t precision; /* TMC_PRECISION_* */
+    uint32_t reserved; /* 0 */
 } tmc_options;
 
 typedef struct tmc_run_stats {
@@ -78,6 +88,8 @@ typedef struct tmc_run_stats {
     double total_ms; /* whole call: clears, kernel, copies back */
     uint32_t grid, block, chunk; /* launch shape used */
     uint32_t sampler;
+    uint32_t precision;
+    uint32_t reserved;
 } tmc_run_stats;
 
 /* Default options (analytic sampler, automatic launch shape, photon 0). */
@@ -95,8 +107,9 @@ int tiny_mc_gpu_run(const tmc_params* p, uint64_t seed, double* heat, double* he
  * tally has 4 * p->shells words; words [2s, 2s+1] are the (low, high) halves
  * of the 128-bit sum of w * 2^TMC_HEAT_FRAC_BITS over deposits in shell s and
  * words [2S + 2s, 2S + 2s + 1] those of (w*w) * 2^TMC_HEAT2_FRAC_BITS
- * (w in binary32; both images are exact).  Tallies of disjoint photon ranges
- * of one stream add as integers.  stats may be NULL. */
+ * (w in binary32: both images are exact; TMC_PRECISION_FP64: rounded to
+ * nearest).  Tallies of disjoint photon ranges of one stream add as
+ * integers.  stats may be NULL. */
 int tiny_mc_gpu_run_ex(const tmc_params* p, uint64_t seed, const tmc_options* opt, uint64_t* tally,
                        tmc_run_stats* stats);
 

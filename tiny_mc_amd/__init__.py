@@ -20,6 +20,10 @@ from ._lib import (  # noqa: F401
     HEAT2_FRAC_BITS,
     HEAT_FRAC_BITS,
     MAX_SHELLS,
+    PRECISION_FP32,
+    PRECISION_FP32_COMPENSATED,
+    PRECISION_FP32_WEIGHT,
+    PRECISION_FP64,
     SAMPLER_ANALYTIC,
     SAMPLER_REJECTION,
     Params,

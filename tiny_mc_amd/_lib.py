@@ -16,6 +16,10 @@ HEAT2_FRAC_BITS = 43
 MAX_SHELLS = 8192
 SAMPLER_ANALYTIC = 0
 SAMPLER_REJECTION = 1
+PRECISION_FP32 = 0
+PRECISION_FP32_WEIGHT = 1
+PRECISION_FP32_COMPENSATED = 2
+PRECISION_FP64 = 3
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 lib_path = os.path.join(_HERE, "libtiny_mc_gpu.so")
@@ -57,6 +61,8 @@ class _Options(ctypes.Structure):
         ("chunk", ctypes.c_uint32),
         ("first_photon", ctypes.c_uint64),
         ("max_interactions", ctypes.c_uint64),
+        ("precision", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -71,6 +77,8 @@ class _Stats(ctypes.Structure):
         ("block", ctypes.c_uint32),
         ("chunk", ctypes.c_uint32),
         ("sampler", ctypes.c_uint32),
+        ("precision", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -99,6 +107,7 @@ class RunStats:
     block: int
     chunk: int
     sampler: int
+    precision: int
 
 
 _lib = None
@@ -158,17 +167,17 @@ def run(params: Params, seed: int):
 
 
 def run_raw(params: Params, seed: int, *, sampler=SAMPLER_ANALYTIC, first_photon=0, max_interactions=0,
-            block=0, grid=0, chunk=0):
+            block=0, grid=0, chunk=0, precision=PRECISION_FP32):
     """Integer tallies (uint64 array of 4*shells: 128-bit (lo, hi) sums) and RunStats."""
     lib = load()
     tally = np.zeros(4 * params.shells, dtype=np.uint64)
-    opt = _Options(sampler, block, grid, chunk, first_photon, max_interactions)
+    opt = _Options(sampler, block, grid, chunk, first_photon, max_interactions, precision, 0)
     st = _Stats()
     c = params._c()
     _check(lib.tiny_mc_gpu_run_ex(ctypes.byref(c), seed, ctypes.byref(opt), _ptr(tally, ctypes.c_uint64),
                                   ctypes.byref(st)), "tiny_mc_gpu_run_ex")
     stats = RunStats(st.photons, st.interactions, st.truncated, st.kernel_ms, st.total_ms, st.grid, st.block,
-                     st.chunk, st.sampler)
+                     st.chunk, st.sampler, st.precision)
     return tally, stats
 
 

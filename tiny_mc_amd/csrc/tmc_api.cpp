@@ -208,7 +208,7 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
 {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t S = p->shells;
-    if (o->sampler > TMC_SAMPLER_REJECTION)
+    if (o->sampler > TMC_SAMPLER_REJECTION || o->precision > TMC_PRECISION_FP64)
         return TMC_E_INVALID;
     if (p->photons > UINT64_MAX - o->first_photon)
         return TMC_E_RANGE;
@@ -222,7 +222,7 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     uint32_t grid = o->grid;
     if (grid == 0) {
         int per_cu = 0;
-        if (tmc::walk_occupancy(o->sampler, block, S, &per_cu) != hipSuccess || per_cu <= 0)
+        if (tmc::walk_occupancy(o->sampler, o->precision, block, S, &per_cu) != hipSuccess || per_cu <= 0)
             return TMC_E_HIP;
         grid = (uint32_t)per_cu * (uint32_t)c->compute_units;
     }
@@ -247,6 +247,16 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     a.shell_max = (float)(S - 1);
     a.shells = S;
     a.chunk = chunk;
+    /* precision variants: constants in FP64, each rounded once */
+    const double mu_a = p->mu_a, mu_s = p->mu_s;
+    a.albedo_d = mu_s / (mu_s + mu_a);
+    a.spm_d = 1e4 / (double)p->microns_per_shell / (mu_a + mu_s);
+    const double l2a = std::log2(a.albedo_d), l2s = -std::log2((double)0.1f);
+    a.l2a_hi = (float)l2a;
+    a.l2a_lo = (float)(l2a - (double)a.l2a_hi);
+    a.l2s_hi = (float)l2s;
+    a.l2s_lo = (float)(l2s - (double)a.l2s_hi);
+    a.spm_w = (float)a.spm_d;
     /* Each wave swaps the workgroup's LDS sums out at least every
      * flush_interval steps, so between two swaps of one entry at most
      * waves_per_wg * 64 * flush_interval deposits of at most 2^43 (heat2
@@ -267,7 +277,7 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     if (p->photons > 0) {
         if (hipEventRecord(c->ev_start, c->stream) != hipSuccess)
             return TMC_E_HIP;
-        if (tmc::launch_walk(o->sampler, a, grid, block, c->stream) != hipSuccess)
+        if (tmc::launch_walk(o->sampler, o->precision, a, grid, block, c->stream) != hipSuccess)
             return TMC_E_HIP;
         if (hipEventRecord(c->ev_stop, c->stream) != hipSuccess)
             return TMC_E_HIP;
@@ -290,6 +300,7 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
         st->block = block;
         st->chunk = chunk;
         st->sampler = o->sampler;
+        st->precision = o->precision;
     }
     return TMC_OK;
 }

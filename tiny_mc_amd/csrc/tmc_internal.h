@@ -24,13 +24,22 @@ struct WalkArgs {
     uint32_t pad;
     unsigned long long* counters; /* [0] claim offset, [1] interactions, [2] truncated */
     unsigned long long* tally; /* 4*shells words of 128-bit sums (lo, hi) */
+    /* precision variants */
+    double albedo_d; /* mu_s / (mu_s + mu_a) in FP64 */
+    double spm_d; /* 1e4 / microns_per_shell / (mu_a + mu_s) in FP64 */
+    float l2a_hi, l2a_lo; /* log2(albedo_d) as a float pair */
+    float l2s_hi, l2s_lo; /* -log2((double)0.1f) as a float pair */
+    float spm_w; /* spm_d rounded to float */
+    uint32_t pad2;
 };
 
-/* Launches the persistent walk kernel on `stream`.  lds_bytes = 16*shells. */
-hipError_t launch_walk(uint32_t sampler, const WalkArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
+/* Launches the persistent walk kernel of `precision` (TMC_PRECISION_*) on
+ * `stream`.  lds_bytes = 16*shells. */
+hipError_t launch_walk(uint32_t sampler, uint32_t precision, const WalkArgs& a, uint32_t grid, uint32_t block,
+                       hipStream_t stream);
 
 /* Resident workgroups per CU for the walk kernel at this block size. */
-hipError_t walk_occupancy(uint32_t sampler, uint32_t block, uint32_t shells, int* wg_per_cu);
+hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t block, uint32_t shells, int* wg_per_cu);
 
 hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out, hipStream_t stream);
 hipError_t launch_eval_math(const float* in, uint32_t n, float* lg, float* sn, float* cs, hipStream_t stream);

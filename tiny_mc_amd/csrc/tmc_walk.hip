@@ -29,6 +29,7 @@
 #include "tmc_math.h"
 #include "tmc_philox.h"
 #include "tmc_step.h"
+#include "tmc_step_prec.h"
 
 namespace tmc {
 
@@ -48,8 +49,32 @@ __device__ __forceinline__ void flush_tallies(unsigned long long* lw, unsigned l
     }
 }
 
-template <uint32_t SAMPLER>
-__global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
+/* One absorption event, by photon state type: the reference's FP32 step or
+ * one of the precision variants (tmc_step_prec.h). */
+template <uint32_t SAMPLER, bool COMP>
+__device__ __forceinline__ bool step_of(Photon& p, const Physics& ph, unsigned long long* lw,
+                                        unsigned long long* lw2)
+{
+    return step<SAMPLER>(p, ph, lw, lw2);
+}
+
+template <uint32_t SAMPLER, bool COMP>
+__device__ __forceinline__ bool step_of(PhotonW& p, const PhysicsW& ph, unsigned long long* lw,
+                                        unsigned long long* lw2)
+{
+    return step<SAMPLER, COMP>(p, ph, lw, lw2);
+}
+
+template <uint32_t SAMPLER, bool COMP>
+__device__ __forceinline__ bool step_of(Photon64& p, const Physics64& ph, unsigned long long* lw,
+                                        unsigned long long* lw2)
+{
+    return step<SAMPLER>(p, ph, lw, lw2);
+}
+
+/* The persistent walk over photons [first_photon, first_photon + photons). */
+template <class PhotonT, class PhysicsT, uint32_t SAMPLER, bool COMP>
+__device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phys)
 {
     extern __shared__ unsigned long long lds_tally[];
     unsigned long long* lw = lds_tally;
@@ -59,8 +84,6 @@ __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
     __syncthreads();
 
     const uint32_t lane = __lane_id();
-    const uint32_t key0 = (uint32_t)a.seed;
-    const uint32_t key1 = (uint32_t)(a.seed >> 32);
 
     /* wave-uniform pool of claimed photon offsets [pool_next, pool_end) */
     uint64_t pool_next = 0, pool_end = 0;
@@ -68,11 +91,9 @@ __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
     uint64_t wave_interactions = 0, wave_truncated = 0;
     uint32_t since_flush = 0;
 
-    const Physics phys = { a.albedo, a.spm, a.shell_max, a.shells - 1, key0, key1 };
-
     /* lane state: one photon (photon.c:11-18) */
     bool alive = false;
-    Photon ph;
+    PhotonT ph;
     launch_photon(ph, 0);
 
     for (;;) {
@@ -124,7 +145,7 @@ __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
 
         bool truncated = false;
         if (alive) {
-            alive = step<SAMPLER>(ph, phys, lw, lw2);
+            alive = step_of<SAMPLER, COMP>(ph, phys, lw, lw2);
             if (alive && ph.k >= a.kcap) {
                 alive = false;
                 truncated = true;
@@ -145,22 +166,62 @@ __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
     }
 }
 
-hipError_t launch_walk(uint32_t sampler, const WalkArgs& a, uint32_t grid, uint32_t block, hipStream_t stream)
+/* The reference's FP32 arithmetic (the production walk). */
+template <uint32_t SAMPLER>
+__global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
+{
+    const Physics phys = { a.albedo, a.spm, a.shell_max, a.shells - 1, (uint32_t)a.seed, (uint32_t)(a.seed >> 32) };
+    walk_body<Photon, Physics, SAMPLER, false>(a, phys);
+}
+
+/* FP32 transport, weight from exact counts (COMP: compensated positions). */
+template <uint32_t SAMPLER, bool COMP>
+__global__ void __launch_bounds__(1024) walk_f32w(WalkArgs a)
+{
+    const PhysicsW phys = { a.l2a_hi, a.l2a_lo, a.l2s_hi, a.l2s_lo, a.spm_w, a.shell_max, a.shells - 1,
+                            (uint32_t)a.seed, (uint32_t)(a.seed >> 32) };
+    walk_body<PhotonW, PhysicsW, SAMPLER, COMP>(a, phys);
+}
+
+/* All-FP64 transport. */
+template <uint32_t SAMPLER>
+__global__ void __launch_bounds__(1024) walk_f64(WalkArgs a)
+{
+    const Physics64 phys = { a.albedo_d, a.spm_d, (double)(a.shells - 1), a.shells - 1, (uint32_t)a.seed,
+                             (uint32_t)(a.seed >> 32) };
+    walk_body<Photon64, Physics64, SAMPLER, false>(a, phys);
+}
+
+namespace {
+
+using WalkKernel = void (*)(WalkArgs);
+
+WalkKernel walk_kernel(uint32_t sampler, uint32_t precision)
+{
+    const bool rej = sampler == 1u;
+    switch (precision) {
+    case 1u: return rej ? walk_f32w<1u, false> : walk_f32w<0u, false>;
+    case 2u: return rej ? walk_f32w<1u, true> : walk_f32w<0u, true>;
+    case 3u: return rej ? walk_f64<1u> : walk_f64<0u>;
+    default: return rej ? walk_f32<1u> : walk_f32<0u>;
+    }
+}
+
+} // namespace
+
+hipError_t launch_walk(uint32_t sampler, uint32_t precision, const WalkArgs& a, uint32_t grid, uint32_t block,
+                       hipStream_t stream)
 {
     const size_t lds = (size_t)a.shells * 16u;
-    if (sampler == 1u)
-        hipLaunchKernelGGL(walk_f32<1u>, dim3(grid), dim3(block), lds, stream, a);
-    else
-        hipLaunchKernelGGL(walk_f32<0u>, dim3(grid), dim3(block), lds, stream, a);
+    hipLaunchKernelGGL(walk_kernel(sampler, precision), dim3(grid), dim3(block), lds, stream, a);
     return hipGetLastError();
 }
 
-hipError_t walk_occupancy(uint32_t sampler, uint32_t block, uint32_t shells, int* wg_per_cu)
+hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t block, uint32_t shells, int* wg_per_cu)
 {
     const size_t lds = (size_t)shells * 16u;
-    if (sampler == 1u)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_f32<1u>, (int)block, lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_f32<0u>, (int)block, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_kernel(sampler, precision), (int)block,
+                                                        lds);
 }
 
 __global__ void philox_kernel(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out)

@@ -40,6 +40,16 @@
 
 namespace tmc {
 
+/* Copies of the overflow shell's sums in each wave's slot (lane & 15 picks
+ * one), after the slot's shell copies. */
+#define TMC_SLOT_HOT 16u
+
+/* 64-bit words of one wave's LDS slot. */
+__host__ __device__ __forceinline__ size_t slot_words(uint32_t shells_max, uint32_t copies)
+{
+    return (size_t)copies * 2u * shells_max + 2u * TMC_SLOT_HOT;
+}
+
 /* Moves one wave's private LDS slot into a set's global sums and zeroes it.
  * Only this wave touches the slot, and a wave's LDS operations complete in
  * order, so plain loads see every earlier ds_add of the wave.  Serving uses
@@ -48,25 +58,36 @@ template <bool SERVE>
 __device__ __forceinline__ void flush_slot(unsigned long long* slot, uint32_t shells_max, uint32_t copies,
                                            uint32_t shells, unsigned long long* g, uint32_t lane)
 {
-    for (uint32_t s = lane; s < shells; s += 64) {
+    unsigned long long* hot = slot + (size_t)copies * 2 * shells_max;
+    for (uint32_t s = lane; s < shells + TMC_SLOT_HOT; s += 64) {
         unsigned long long a = 0, b = 0;
-        for (uint32_t c = 0; c < copies; ++c) {
-            unsigned long long* lw = slot + (size_t)c * 2 * shells_max;
-            a += lw[s];
-            b += lw[shells_max + s];
-            lw[s] = 0;
-            lw[shells_max + s] = 0;
+        uint32_t shell = s;
+        if (s < shells) {
+            for (uint32_t c = 0; c < copies; ++c) {
+                unsigned long long* lw = slot + (size_t)c * 2 * shells_max;
+                a += lw[s];
+                b += lw[shells_max + s];
+                lw[s] = 0;
+                lw[shells_max + s] = 0;
+            }
+        } else {
+            const uint32_t h = s - shells;
+            a = hot[h];
+            b = hot[TMC_SLOT_HOT + h];
+            hot[h] = 0;
+            hot[TMC_SLOT_HOT + h] = 0;
+            shell = shells - 1;
         }
         if (SERVE) {
             if (a)
-                add128_sys(g + 2 * s, a);
+                add128_sys(g + 2 * shell, a);
             if (b)
-                add128_sys(g + 2 * shells + 2 * s, b);
+                add128_sys(g + 2 * shells + 2 * shell, b);
         } else {
             if (a)
-                add128(g + 2 * s, a);
+                add128(g + 2 * shell, a);
             if (b)
-                add128(g + 2 * shells + 2 * s, b);
+                add128(g + 2 * shells + 2 * shell, b);
         }
     }
 }
@@ -283,11 +304,12 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     extern __shared__ unsigned long long lds_slots[];
     const uint32_t lane = __lane_id();
     const uint32_t wave = threadIdx.x >> 6;
-    unsigned long long* slot = lds_slots + (size_t)wave * a.copies * 2 * a.shells_max;
-    for (uint32_t i = lane; i < a.copies * 2 * a.shells_max; i += 64)
+    unsigned long long* slot = lds_slots + (size_t)wave * slot_words(a.shells_max, a.copies);
+    for (uint32_t i = lane; i < slot_words(a.shells_max, a.copies); i += 64)
         slot[i] = 0;
     unsigned long long* lw = slot + (a.copies == 2 && lane >= 32 ? 2 * a.shells_max : 0);
-    unsigned long long* lw2 = lw + a.shells_max;
+    unsigned long long* hot = slot + (size_t)a.copies * 2 * a.shells_max + (lane & (TMC_SLOT_HOT - 1));
+    const LaneTally lt = { lw, lw + a.shells_max, hot, hot + TMC_SLOT_HOT };
     const uint32_t slot_mask = SERVE ? (1u << TMC_SERVE_SLOT_BITS) - 1u : 0xFFFFFFFFu;
 
     SetState st;
@@ -370,7 +392,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
 
         bool truncated = false;
         if (alive) {
-            alive = step<SAMPLER>(ph, st.phys, lw, lw2);
+            alive = step<SAMPLER>(ph, st.phys, lt);
             if (alive && ph.k >= st.kcap) {
                 alive = false;
                 truncated = true;
@@ -416,7 +438,7 @@ __global__ void batch_clear(unsigned long long* tally, uint64_t tally_words, uns
 
 size_t batch_lds_bytes(uint32_t block, uint32_t shells_max, uint32_t copies)
 {
-    return (size_t)(block / 64) * copies * 2 * shells_max * sizeof(unsigned long long);
+    return (size_t)(block / 64) * slot_words(shells_max, copies) * sizeof(unsigned long long);
 }
 
 hipError_t launch_batch(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream)

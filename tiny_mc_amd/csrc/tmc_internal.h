@@ -34,7 +34,7 @@ struct WalkArgs {
 };
 
 /* Launches the persistent walk kernel of `precision` (TMC_PRECISION_*) on
- * `stream`.  lds_bytes = 16*shells. */
+ * `stream`.  LDS: 16 bytes per shell plus the overflow shell's lane copies. */
 hipError_t launch_walk(uint32_t sampler, uint32_t precision, const WalkArgs& a, uint32_t grid, uint32_t block,
                        hipStream_t stream);
 

@@ -39,11 +39,22 @@ __device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon)
     p.phi = (uint32_t)(photon >> 32);
 }
 
+/* LDS words one lane deposits into: the shell sums (w and w*w images) and
+ * the lane's own copy of the overflow shell's sums.  A long walk ends up in
+ * the overflow shell (the last one) most of the time, so there the lanes
+ * of a wave would all hit one word; each lane group has its own copy
+ * instead, and the copies are added up when the sums are flushed. */
+struct LaneTally {
+    unsigned long long* w1;
+    unsigned long long* w2;
+    unsigned long long* hot1;
+    unsigned long long* hot2;
+};
+
 /* Walks one absorption event of p, adding its deposit to the fixed-point
- * shell sums lw / lw2 (LDS).  Returns false when the roulette kills the
- * photon. */
+ * shell sums in LDS.  Returns false when the roulette kills the photon. */
 template <uint32_t SAMPLER>
-__device__ __forceinline__ bool step(Photon& p, const Physics& ph, unsigned long long* lw, unsigned long long* lw2)
+__device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt)
 {
     const tmc_u32x4 r = tmc_philox4x32_10(p.k, 0u, p.plo, p.phi, ph.key0, ph.key1);
 
@@ -56,9 +67,9 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, unsigned long
     /* absorb (photon.c:26-32) */
     const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
     const float f = sqrt_rn(r2) * ph.spm;
-    const uint32_t shell = f < ph.shell_max ? (uint32_t)f : ph.last_shell;
-    atomicAdd(&lw[shell], (unsigned long long)fx_heat(p.weight));
-    atomicAdd(&lw2[shell], (unsigned long long)fx_heat2(p.weight));
+    const bool inner = f < ph.shell_max; /* else the overflow shell, ph.last_shell */
+    atomicAdd(inner ? lt.w1 + (uint32_t)f : lt.hot1, (unsigned long long)fx_heat(p.weight));
+    atomicAdd(inner ? lt.w2 + (uint32_t)f : lt.hot2, (unsigned long long)fx_heat2(p.weight));
     p.weight = p.weight * ph.albedo;
 
     /* new direction (photon.c:34-43) */

@@ -107,7 +107,7 @@ __device__ __forceinline__ void compensated_move(float& x, float& xl, float t, f
 }
 
 template <uint32_t SAMPLER, bool COMP>
-__device__ __forceinline__ bool step(PhotonW& p, const PhysicsW& ph, unsigned long long* lw, unsigned long long* lw2)
+__device__ __forceinline__ bool step(PhotonW& p, const PhysicsW& ph, const LaneTally& lt)
 {
     const tmc_u32x4 r = tmc_philox4x32_10(p.k, 0u, p.plo, p.phi, ph.key0, ph.key1);
 
@@ -126,9 +126,9 @@ __device__ __forceinline__ bool step(PhotonW& p, const PhysicsW& ph, unsigned lo
     /* absorb (photon.c:26-32) */
     const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
     const float f = sqrt_rn(r2) * ph.spm;
-    const uint32_t shell = f < ph.shell_max ? (uint32_t)f : ph.last_shell;
-    atomicAdd(&lw[shell], (unsigned long long)fx_heat(p.weight));
-    atomicAdd(&lw2[shell], (unsigned long long)fx_heat2(p.weight));
+    const bool inner = f < ph.shell_max;
+    atomicAdd(inner ? lt.w1 + (uint32_t)f : lt.hot1, (unsigned long long)fx_heat(p.weight));
+    atomicAdd(inner ? lt.w2 + (uint32_t)f : lt.hot2, (unsigned long long)fx_heat2(p.weight));
     ++p.e;
     p.weight = weight_from_counts(p.e, p.r, ph);
 
@@ -212,8 +212,7 @@ __device__ __forceinline__ uint64_t fx_heat64(double w) { return (uint64_t)__dou
 __device__ __forceinline__ uint64_t fx_heat2_64(double w) { return (uint64_t)__double2ull_rn((w * w) * 0x1p43); }
 
 template <uint32_t SAMPLER>
-__device__ __forceinline__ bool step(Photon64& p, const Physics64& ph, unsigned long long* lw,
-                                     unsigned long long* lw2)
+__device__ __forceinline__ bool step(Photon64& p, const Physics64& ph, const LaneTally& lt)
 {
     const tmc_u32x4 r = tmc_philox4x32_10(p.k, 0u, p.plo, p.phi, ph.key0, ph.key1);
 
@@ -225,9 +224,9 @@ __device__ __forceinline__ bool step(Photon64& p, const Physics64& ph, unsigned 
 
     /* absorb (photon.c:26-32) */
     const double f = sqrt(p.x * p.x + p.y * p.y + p.z * p.z) * ph.spm;
-    const uint32_t shell = f < ph.shell_max ? (uint32_t)f : ph.last_shell;
-    atomicAdd(&lw[shell], (unsigned long long)fx_heat64(p.weight));
-    atomicAdd(&lw2[shell], (unsigned long long)fx_heat2_64(p.weight));
+    const bool inner = f < ph.shell_max;
+    atomicAdd(inner ? lt.w1 + (uint32_t)f : lt.hot1, (unsigned long long)fx_heat64(p.weight));
+    atomicAdd(inner ? lt.w2 + (uint32_t)f : lt.hot2, (unsigned long long)fx_heat2_64(p.weight));
     p.weight *= ph.albedo;
 
     /* new direction (photon.c:34-43) */

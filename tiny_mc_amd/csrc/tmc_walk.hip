@@ -33,10 +33,19 @@
 
 namespace tmc {
 
+/* Copies of the overflow shell's sums per workgroup (lane & 31 picks one):
+ * 32 consecutive 64-bit words cover the LDS banks once. */
+#define TMC_WALK_HOT 32u
+
+/* LDS words of one workgroup: w and w*w sums per shell, then the
+ * overflow shell's copies of both. */
+__host__ __device__ __forceinline__ size_t walk_lds_words(uint32_t shells) { return 2u * shells + 2u * TMC_WALK_HOT; }
+
 /* Moves this workgroup's LDS tallies into the global 128-bit sums.  Safe
  * while other waves of the workgroup keep adding: each entry is taken by an
  * atomic swap with zero, so every addition is counted exactly once. */
 __device__ __forceinline__ void flush_tallies(unsigned long long* lw, unsigned long long* lw2,
+                                              unsigned long long* hot1, unsigned long long* hot2,
                                               unsigned long long* g, uint32_t shells, uint32_t lane)
 {
     for (uint32_t s = lane; s < shells; s += 64) {
@@ -47,29 +56,35 @@ __device__ __forceinline__ void flush_tallies(unsigned long long* lw, unsigned l
         if (b)
             add128(g + 2 * shells + 2 * s, b);
     }
+    if (lane < TMC_WALK_HOT) {
+        const uint32_t last = shells - 1;
+        const unsigned long long a = atomicExch(&hot1[lane], 0ull);
+        const unsigned long long b = atomicExch(&hot2[lane], 0ull);
+        if (a)
+            add128(g + 2 * last, a);
+        if (b)
+            add128(g + 2 * shells + 2 * last, b);
+    }
 }
 
 /* One absorption event, by photon state type: the reference's FP32 step or
  * one of the precision variants (tmc_step_prec.h). */
 template <uint32_t SAMPLER, bool COMP>
-__device__ __forceinline__ bool step_of(Photon& p, const Physics& ph, unsigned long long* lw,
-                                        unsigned long long* lw2)
+__device__ __forceinline__ bool step_of(Photon& p, const Physics& ph, const LaneTally& lt)
 {
-    return step<SAMPLER>(p, ph, lw, lw2);
+    return step<SAMPLER>(p, ph, lt);
 }
 
 template <uint32_t SAMPLER, bool COMP>
-__device__ __forceinline__ bool step_of(PhotonW& p, const PhysicsW& ph, unsigned long long* lw,
-                                        unsigned long long* lw2)
+__device__ __forceinline__ bool step_of(PhotonW& p, const PhysicsW& ph, const LaneTally& lt)
 {
-    return step<SAMPLER, COMP>(p, ph, lw, lw2);
+    return step<SAMPLER, COMP>(p, ph, lt);
 }
 
 template <uint32_t SAMPLER, bool COMP>
-__device__ __forceinline__ bool step_of(Photon64& p, const Physics64& ph, unsigned long long* lw,
-                                        unsigned long long* lw2)
+__device__ __forceinline__ bool step_of(Photon64& p, const Physics64& ph, const LaneTally& lt)
 {
-    return step<SAMPLER>(p, ph, lw, lw2);
+    return step<SAMPLER>(p, ph, lt);
 }
 
 /* The persistent walk over photons [first_photon, first_photon + photons). */
@@ -79,11 +94,14 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
     extern __shared__ unsigned long long lds_tally[];
     unsigned long long* lw = lds_tally;
     unsigned long long* lw2 = lds_tally + a.shells;
-    for (uint32_t i = threadIdx.x; i < 2 * a.shells; i += blockDim.x)
+    unsigned long long* hot1 = lds_tally + 2 * a.shells;
+    unsigned long long* hot2 = hot1 + TMC_WALK_HOT;
+    for (uint32_t i = threadIdx.x; i < walk_lds_words(a.shells); i += blockDim.x)
         lds_tally[i] = 0;
     __syncthreads();
 
     const uint32_t lane = __lane_id();
+    const LaneTally lt = { lw, lw2, hot1 + (lane & (TMC_WALK_HOT - 1)), hot2 + (lane & (TMC_WALK_HOT - 1)) };
 
     /* wave-uniform pool of claimed photon offsets [pool_next, pool_end) */
     uint64_t pool_next = 0, pool_end = 0;
@@ -145,7 +163,7 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
 
         bool truncated = false;
         if (alive) {
-            alive = step_of<SAMPLER, COMP>(ph, phys, lw, lw2);
+            alive = step_of<SAMPLER, COMP>(ph, phys, lt);
             if (alive && ph.k >= a.kcap) {
                 alive = false;
                 truncated = true;
@@ -155,10 +173,10 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
 
         if (++since_flush >= a.flush_interval) {
             since_flush = 0;
-            flush_tallies(lw, lw2, a.tally, a.shells, lane);
+            flush_tallies(lw, lw2, hot1, hot2, a.tally, a.shells, lane);
         }
     }
-    flush_tallies(lw, lw2, a.tally, a.shells, lane);
+    flush_tallies(lw, lw2, hot1, hot2, a.tally, a.shells, lane);
     if (lane == 0) {
         atomicAdd(a.counters + 1, (unsigned long long)wave_interactions);
         if (wave_truncated)
@@ -212,14 +230,14 @@ WalkKernel walk_kernel(uint32_t sampler, uint32_t precision)
 hipError_t launch_walk(uint32_t sampler, uint32_t precision, const WalkArgs& a, uint32_t grid, uint32_t block,
                        hipStream_t stream)
 {
-    const size_t lds = (size_t)a.shells * 16u;
+    const size_t lds = walk_lds_words(a.shells) * sizeof(unsigned long long);
     hipLaunchKernelGGL(walk_kernel(sampler, precision), dim3(grid), dim3(block), lds, stream, a);
     return hipGetLastError();
 }
 
 hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t block, uint32_t shells, int* wg_per_cu)
 {
-    const size_t lds = (size_t)shells * 16u;
+    const size_t lds = walk_lds_words(shells) * sizeof(unsigned long long);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_kernel(sampler, precision), (int)block,
                                                         lds);
 }

@@ -142,6 +142,12 @@ const char* tiny_mc_gpu_strerror(int status);
 int tiny_mc_gpu_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out);
 int tiny_mc_gpu_eval_math(const float* in, uint32_t n, float* log_out, float* sin_out, float* cos_out);
 
+/* Diagnostic: compares the walk's short correctly rounded square root with
+ * the device's IEEE square root on every float whose bit pattern lies in
+ * [first_bits, first_bits + count); returns the number of mismatches and
+ * the lowest mismatching pattern (0 if none). */
+int tiny_mc_gpu_check_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismatches, uint32_t* first_mismatch);
+
 /* ------------------------------------------------------------------------
  * Batched sweep: many parameter sets in one persistent launch.
  *

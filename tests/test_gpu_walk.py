@@ -130,3 +130,12 @@ def test_vs_fp64_oracle_2e24(tmc, orc, sampler):
     assert st.interactions == on
     # and the aggregate is rounding-level close
     assert abs(heat.sum() / oh.sum() - 1.0) < 1e-6
+
+
+def test_short_sqrt_exhaustive(tmc):
+    """The walk's short correctly rounded sqrt equals the IEEE sqrt on every
+    float of its domain: +0 and [2^-96, FLT_MAX] (all 1.8e9 patterns)."""
+    lo = int(np.float32(2.0 ** -96).view(np.uint32))
+    hi = int(np.float32(np.finfo(np.float32).max).view(np.uint32))
+    assert tmc.check_sqrt(0, 1) == (0, 0)
+    assert tmc.check_sqrt(lo, hi - lo + 1) == (0, 0)

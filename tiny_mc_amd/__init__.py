@@ -29,6 +29,7 @@ from ._lib import (  # noqa: F401
     Params,
     RunStats,
     TmcError,
+    check_sqrt,
     device_info,
     eval_math,
     lib_path,

@@ -139,8 +139,9 @@ def load():
     fp = P(ctypes.c_float)
     lib.tiny_mc_gpu_philox.argtypes = [u32p, u32p, ctypes.c_uint32, u32p]
     lib.tiny_mc_gpu_eval_math.argtypes = [fp, ctypes.c_uint32, fp, fp, fp]
+    lib.tiny_mc_gpu_check_sqrt.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u64p, P(ctypes.c_uint32)]
     for name in ("tiny_mc_gpu_run", "tiny_mc_gpu_run_ex", "tiny_mc_gpu_tally_to_heat", "tiny_mc_gpu_init",
-                 "tiny_mc_gpu_device_info", "tiny_mc_gpu_philox", "tiny_mc_gpu_eval_math"):
+                 "tiny_mc_gpu_device_info", "tiny_mc_gpu_philox", "tiny_mc_gpu_eval_math", "tiny_mc_gpu_check_sqrt"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -204,6 +205,17 @@ def philox(ctr, key):
     _check(lib.tiny_mc_gpu_philox(_ptr(ctr, ctypes.c_uint32), _ptr(key, ctypes.c_uint32), ctr.shape[0],
                                   _ptr(out, ctypes.c_uint32)), "tiny_mc_gpu_philox")
     return out
+
+
+def check_sqrt(first_bits, count):
+    """Mismatches of the walk's short square root against the device IEEE
+    sqrt over float bit patterns [first_bits, first_bits + count):
+    (count, lowest mismatching pattern)."""
+    lib = load()
+    n = ctypes.c_uint64()
+    f = ctypes.c_uint32()
+    _check(lib.tiny_mc_gpu_check_sqrt(first_bits, count, ctypes.byref(n), ctypes.byref(f)), "tiny_mc_gpu_check_sqrt")
+    return n.value, f.value
 
 
 def eval_math(x):

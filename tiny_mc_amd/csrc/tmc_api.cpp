@@ -490,3 +490,27 @@ TMC_EXPORT int tiny_mc_gpu_eval_math(const float* in, uint32_t n, float* log_out
         return TMC_E_HIP;
     return TMC_OK;
 }
+
+TMC_EXPORT int tiny_mc_gpu_check_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismatches,
+                                      uint32_t* first_mismatch)
+{
+    if (!mismatches || !first_mismatch || count > (1ull << 32) || (uint64_t)first_bits + count > (1ull << 32))
+        return TMC_E_INVALID;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Context* c = nullptr;
+    int rc = tmc::acquire_context(&c);
+    if (rc != TMC_OK)
+        return rc;
+    DevBufs b;
+    unsigned long long h[2] = { 0ull, ~0ull };
+    if (hipMalloc(&b.p[0], sizeof h) != hipSuccess)
+        return TMC_E_NOMEM;
+    if (hipMemcpyAsync(b.p[0], h, sizeof h, hipMemcpyHostToDevice, c->stream) != hipSuccess
+        || tmc::launch_check_sqrt(first_bits, count, (unsigned long long*)b.p[0], c->stream) != hipSuccess
+        || hipMemcpyAsync(h, b.p[0], sizeof h, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipStreamSynchronize(c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    *mismatches = h[0];
+    *first_mismatch = h[0] ? (uint32_t)h[1] : 0u;
+    return TMC_OK;
+}

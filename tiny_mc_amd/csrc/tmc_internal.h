@@ -42,6 +42,7 @@ hipError_t launch_walk(uint32_t sampler, uint32_t precision, const WalkArgs& a, 
 hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t block, uint32_t shells, int* wg_per_cu);
 
 hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out, hipStream_t stream);
+hipError_t launch_check_sqrt(uint32_t first, uint64_t count, unsigned long long* out, hipStream_t stream);
 hipError_t launch_eval_math(const float* in, uint32_t n, float* lg, float* sn, float* cs, hipStream_t stream);
 
 } // namespace tmc

@@ -78,6 +78,39 @@ __device__ __forceinline__ void sincos2pi32(float a, float* s_out, float* c_out)
  * sqrtf under -fhip-fp32-correctly-rounded-divide-sqrt. */
 __device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 
+/* Correctly rounded square root for x == 0 or 2^-96 <= x < inf, which covers
+ * every radicand of the walk but the radius of a near-exact cancellation.
+ * The hardware estimate (v_sqrt_f32, within one ulp) is corrected by one
+ * ulp from the signs of the residuals x - s'*s of its two neighbours s'
+ * (exact by FMA): the step of the compiler's IEEE lowering without its
+ * scaling of tiny inputs and its infinity / NaN screening (x = 0: the lower
+ * neighbour's residual is NaN and the upper one's is 0, so 0 stays).  Equal
+ * to sqrt_rn on every float of the domain (exhaustive device check,
+ * tiny_mc_gpu_check_sqrt). */
+__device__ __forceinline__ float sqrt_cr_walk(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    const float y = rd <= 0.0f ? sd : s;
+    return ru > 0.0f ? su : y;
+}
+
+/* Smallest positive radicand sqrt_cr_walk is exact for. */
+#define TMC_SQRT_WALK_MIN 0x1p-96f
+
+/* Correctly rounded radius sqrt(r2) of a walk position: the short form
+ * unless some lane of the wave has a radicand below its domain (never seen:
+ * it needs a cancellation to within 2^-48 of the origin). */
+__device__ __forceinline__ float sqrt_radius(float r2)
+{
+    if (__builtin_expect(__ballot(r2 > 0.0f && r2 < TMC_SQRT_WALK_MIN) != 0, 0))
+        return sqrt_rn(r2);
+    return sqrt_cr_walk(r2);
+}
+
 /* Fixed-point images of one deposit: w * 2^33 and (w*w) * 2^43, truncated.
  * Exact for every w in [2^-10, 1]; the walk deposits only weights in
  * [0.001, 1] (weights below the 0.001 cutoff are rouletted before the next

@@ -66,7 +66,7 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
 
     /* absorb (photon.c:26-32) */
     const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
-    const float f = sqrt_rn(r2) * ph.spm;
+    const float f = sqrt_radius(r2) * ph.spm;
     const bool inner = f < ph.shell_max; /* else the overflow shell, ph.last_shell */
     atomicAdd(inner ? lt.w1 + (uint32_t)f : lt.hot1, (unsigned long long)fx_heat(p.weight));
     atomicAdd(inner ? lt.w2 + (uint32_t)f : lt.hot2, (unsigned long long)fx_heat2(p.weight));
@@ -95,12 +95,12 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
             }
         }
         p.u = __builtin_fmaf(2.0f, tt, -1.0f);
-        const float s = 2.0f * sqrt_rn(1.0f - tt);
+        const float s = 2.0f * sqrt_cr_walk(1.0f - tt); /* 1 - tt: 0 or >= 2^-24 */
         p.v = xi1 * s;
         p.w = xi2 * s;
     } else {
         const float mu = __builtin_fmaf(2.0f, tmc_u01(r.x1), -1.0f);
-        const float s = sqrt_rn(__builtin_fmaf(-mu, mu, 1.0f));
+        const float s = sqrt_cr_walk(__builtin_fmaf(-mu, mu, 1.0f)); /* 0 or >= 2^-23 */
         float sn, cs;
         sincos2pi32(tmc_u01(r.x2), &sn, &cs);
         p.u = mu;

@@ -268,6 +268,35 @@ __global__ void eval_math_kernel(const float* in, uint32_t n, float* lg, float* 
     cs[i] = c;
 }
 
+/* Compares sqrt_cr_walk with the IEEE square root on the floats with bit
+ * patterns [first, first + count): out[0] += mismatches, out[1] = min of the
+ * mismatching patterns. */
+__global__ void check_sqrt_kernel(uint32_t first, uint64_t count, unsigned long long* out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bad = 0, lowest = ~0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint32_t bits = first + (uint32_t)i;
+        const float x = __uint_as_float(bits);
+        if (__float_as_uint(sqrt_cr_walk(x)) != __float_as_uint(sqrt_rn(x))) {
+            ++bad;
+            lowest = bits < lowest ? bits : lowest;
+        }
+    }
+    if (bad) {
+        atomicAdd(&out[0], bad);
+        atomicMin(&out[1], lowest);
+    }
+}
+
+hipError_t launch_check_sqrt(uint32_t first, uint64_t count, unsigned long long* out, hipStream_t stream)
+{
+    if (count == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(check_sqrt_kernel, dim3(8192), dim3(256), 0, stream, first, count, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out, hipStream_t stream)
 {
     if (n == 0)

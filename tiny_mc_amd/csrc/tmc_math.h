@@ -27,6 +27,32 @@ namespace tmc {
 #define TMC_LN2_LO 0x1.7f7d1cp-20f
 #define TMC_SQRT_HALF 0x1.6a09e6p-1f
 
+/* log32 of x = m * 2^-24 for an integer-valued float m >= 1, taking m:
+ * frexp of x and of m differ only in the exponent (by 24), so this equals
+ * log32(m * 0x1p-24f) bit for bit without the scaling. */
+__device__ __forceinline__ float log32_u24(float m)
+{
+    int e = __builtin_amdgcn_frexp_expf(m) - 24;
+    float f = __builtin_amdgcn_frexp_mantf(m);
+    if (f < TMC_SQRT_HALF) {
+        f = f + f;
+        e -= 1;
+    }
+    const float g = f - 1.0f;
+    float q = -0x1.38301cp-4f;
+    q = __builtin_fmaf(q, g, 0x1.084864p-3f);
+    q = __builtin_fmaf(q, g, -0x1.0f3766p-3f);
+    q = __builtin_fmaf(q, g, 0x1.22719cp-3f);
+    q = __builtin_fmaf(q, g, -0x1.54276ep-3f);
+    q = __builtin_fmaf(q, g, 0x1.99a3f2p-3f);
+    q = __builtin_fmaf(q, g, -0x1.000426p-2f);
+    q = __builtin_fmaf(q, g, 0x1.555550p-2f);
+    q = __builtin_fmaf(q, g, -0x1.fffff8p-2f);
+    const float l = __builtin_fmaf(g * g, q, g);
+    const float ef = (float)e;
+    return __builtin_fmaf(ef, TMC_LN2_HI, __builtin_fmaf(ef, TMC_LN2_LO, l));
+}
+
 __device__ __forceinline__ float log32(float x)
 {
     int e = __builtin_amdgcn_frexp_expf(x);
@@ -102,11 +128,12 @@ __device__ __forceinline__ float sqrt_cr_walk(float x)
 #define TMC_SQRT_WALK_MIN 0x1p-96f
 
 /* Correctly rounded radius sqrt(r2) of a walk position: the short form
- * unless some lane of the wave has a radicand below its domain (never seen:
- * it needs a cancellation to within 2^-48 of the origin). */
+ * unless some lane of the wave has a radicand below 2^-96 (a zero-length
+ * first step, probability 2^-24 per photon, or a cancellation to within
+ * 2^-48 of the origin, never seen), where the IEEE lowering is used. */
 __device__ __forceinline__ float sqrt_radius(float r2)
 {
-    if (__builtin_expect(__ballot(r2 > 0.0f && r2 < TMC_SQRT_WALK_MIN) != 0, 0))
+    if (__builtin_expect(__ballot(r2 < TMC_SQRT_WALK_MIN) != 0, 0))
         return sqrt_rn(r2);
     return sqrt_cr_walk(r2);
 }
@@ -119,8 +146,18 @@ __device__ __forceinline__ float sqrt_radius(float r2)
 #define TMC_HEAT_FRAC_BITS 33
 #define TMC_HEAT2_FRAC_BITS 43
 
-__device__ __forceinline__ uint64_t fx_heat(float w) { return (uint64_t)(w * 0x1p33f); }
-__device__ __forceinline__ uint64_t fx_heat2(float w) { return (uint64_t)((w * w) * 0x1p43f); }
+/* floor(x) as a 64-bit integer for 0 <= x < 2^44: the high word floor(x / 2^32)
+ * and the low word x - hi * 2^32 are exact in binary32 (x >= 2^32 is an
+ * integer), and the low word's conversion truncates. */
+__device__ __forceinline__ uint64_t floor_u64_small(float x)
+{
+    const float hi = __builtin_floorf(x * 0x1p-32f);
+    const float lo = __builtin_fmaf(hi, -0x1p32f, x);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo;
+}
+
+__device__ __forceinline__ uint64_t fx_heat(float w) { return floor_u64_small(w * 0x1p33f); }
+__device__ __forceinline__ uint64_t fx_heat2(float w) { return floor_u64_small((w * w) * 0x1p43f); }
 
 } // namespace tmc
 

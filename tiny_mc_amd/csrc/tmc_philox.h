@@ -63,4 +63,10 @@ TMC_HD float tmc_u01(uint32_t r)
     return (float)((r >> 8) + 1u) * 0x1p-24f;
 }
 
+/* The same uniform times 2^24: the integer (r >> 8) + 1 as a float. */
+TMC_HD float tmc_u24(uint32_t r)
+{
+    return (float)((r >> 8) + 1u);
+}
+
 #endif

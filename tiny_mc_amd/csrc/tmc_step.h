@@ -59,7 +59,7 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
     const tmc_u32x4 r = tmc_philox4x32_10(p.k, 0u, p.plo, p.phi, ph.key0, ph.key1);
 
     /* move (photon.c:21-24) */
-    const float t = -log32(tmc_u01(r.x0));
+    const float t = -log32_u24(tmc_u24(r.x0)); /* = -log32(tmc_u01(r.x0)) */
     p.x = __builtin_fmaf(t, p.u, p.x);
     p.y = __builtin_fmaf(t, p.v, p.y);
     p.z = __builtin_fmaf(t, p.w, p.z);
@@ -79,8 +79,8 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
         float xi1, xi2, tt;
         uint32_t att = 0;
         for (;;) {
-            xi1 = __builtin_fmaf(2.0f, tmc_u01(w0), -1.0f);
-            xi2 = __builtin_fmaf(2.0f, tmc_u01(w1), -1.0f);
+            xi1 = __builtin_fmaf(tmc_u24(w0), 0x1p-23f, -1.0f);
+            xi2 = __builtin_fmaf(tmc_u24(w1), 0x1p-23f, -1.0f);
             tt = __builtin_fmaf(xi2, xi2, xi1 * xi1);
             ++att;
             if ((tt <= 1.0f && tt > 0.0f) || att >= 64u)
@@ -99,7 +99,7 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
         p.v = xi1 * s;
         p.w = xi2 * s;
     } else {
-        const float mu = __builtin_fmaf(2.0f, tmc_u01(r.x1), -1.0f);
+        const float mu = __builtin_fmaf(tmc_u24(r.x1), 0x1p-23f, -1.0f); /* = 2 u - 1, both exact */
         const float s = sqrt_cr_walk(__builtin_fmaf(-mu, mu, 1.0f)); /* 0 or >= 2^-23 */
         float sn, cs;
         sincos2pi32(tmc_u01(r.x2), &sn, &cs);

@@ -53,7 +53,8 @@ __host__ __device__ __forceinline__ size_t slot_words(uint32_t shells_max, uint3
 /* Moves one wave's private LDS slot into a set's global sums and zeroes it.
  * Only this wave touches the slot, and a wave's LDS operations complete in
  * order, so plain loads see every earlier ds_add of the wave.  Serving uses
- * system-scope atomics, so the sums live where the host's copies read them. */
+ * system-scope atomics, so the wave that finishes the request reads every
+ * wave's additions when it takes the sums. */
 template <bool SERVE>
 __device__ __forceinline__ void flush_slot(unsigned long long* slot, uint32_t shells_max, uint32_t copies,
                                            uint32_t shells, unsigned long long* g, uint32_t lane)

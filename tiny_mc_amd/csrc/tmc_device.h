@@ -28,8 +28,9 @@ __device__ __forceinline__ void add128(unsigned long long* p, unsigned long long
         atomicAdd(p + 1, 1ull);
 }
 
-/* add128 with system-scope atomics: performed where host copies read
- * (memory), not in this XCD's L2. */
+/* add128 with system-scope atomics, performed at one point for every XCD,
+ * so the wave that later exchanges the sum with zero reads every wave's
+ * additions. */
 __device__ __forceinline__ void add128_sys(unsigned long long* p, unsigned long long v)
 {
     const unsigned long long old = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);

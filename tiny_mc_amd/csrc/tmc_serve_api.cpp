@@ -271,9 +271,11 @@ void serve_step(tmc_server* s)
     }
 
     /* publish, shortest remaining work first, keeping about one unclaimed
-     * chunk per resident wave.  A ring index is rewritten only when it is
-     * more than ring - 2 * waves below the claims reported, so every wave
-     * has read it (at most one ticket per wave is outstanding). */
+     * chunk per resident wave.  A ring index is rewritten only after the
+     * tail has moved more than ring - 2 * waves past the highest claim
+     * reported.  A wave reads its entry at the first poll after the tail
+     * passes its ticket (within 16 of its steps), while rewriting an entry
+     * it has not read would take publishing about a million chunks first. */
     s->claimed = std::max<uint64_t>(s->claimed, load_acquire(&s->h_ctl->claimed));
     const uint64_t backlog_max = std::min<uint64_t>(ring - 2 * s->total_waves, s->total_waves);
     auto unclaimed = [s]() -> uint64_t { return s->published > s->claimed ? s->published - s->claimed : 0; };

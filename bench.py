@@ -167,6 +167,7 @@ def sweep_leg():
         "one_launch_per_set_photons_per_s": photons / per_set_s,
         "speedup_vs_one_launch_per_set": per_set_s / batched_s,
         "chunks": st.chunks,
+        "lane_utilisation": st.lane_utilisation,
         "launch": {"grid": st.grid, "block": st.block, "waves_per_simd": st.waves_per_simd,
                    "copies": st.copies, "chunk_work": st.chunk_work, "hip_graph": True},
     }

@@ -187,6 +187,9 @@ typedef struct tmc_batch_stats {
     double kernel_ms; /* clear + walk on the device (HIP events) */
     double total_ms; /* whole call */
     uint32_t grid, block, copies, waves_per_simd;
+    uint64_t idle_lane_steps; /* lanes without a photon while their wave stepped
+                                 (drains at set switches and the end of the queue):
+                                 lane utilisation = interactions / (interactions + idle) */
 } tmc_batch_stats;
 
 typedef struct tmc_batch_plan tmc_batch_plan;

@@ -29,6 +29,9 @@ def test_batch_equals_sets_alone(tmc, sampler):
     for i, (a, b) in enumerate(zip(parts, ref)):
         np.testing.assert_array_equal(a, b, err_msg=f"set {i}")
     assert st.sets == len(sets) and st.photons == sum(s[0].photons for s in sets)
+    # every stepping iteration of a wave accounts for its 64 lanes
+    assert (st.interactions + st.idle_lane_steps) % 64 == 0
+    assert 0.0 < st.lane_utilisation <= 1.0
 
 
 def test_batch_equals_mirror(tmc, orc):

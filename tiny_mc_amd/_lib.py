@@ -279,6 +279,7 @@ class _BatchStats(ctypes.Structure):
         ("block", ctypes.c_uint32),
         ("copies", ctypes.c_uint32),
         ("waves_per_simd", ctypes.c_uint32),
+        ("idle_lane_steps", ctypes.c_uint64),
     ]
 
 
@@ -297,6 +298,13 @@ class BatchStats:
     block: int
     copies: int
     waves_per_simd: int
+    idle_lane_steps: int
+
+    @property
+    def lane_utilisation(self):
+        """Share of lane-steps that walked a photon while waves stepped."""
+        total = self.interactions + self.idle_lane_steps
+        return self.interactions / total if total else 0.0
 
 
 @dataclass
@@ -350,7 +358,7 @@ def _sets_array(sets):
 def _batch_stats(st):
     return BatchStats(st.sets, st.photons, st.interactions, st.truncated, st.chunks, st.chunk_work,
                       st.expected_interactions, st.kernel_ms, st.total_ms, st.grid, st.block, st.copies,
-                      st.waves_per_simd)
+                      st.waves_per_simd, st.idle_lane_steps)
 
 
 def split_tallies(sets, flat):

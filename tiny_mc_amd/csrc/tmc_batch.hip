@@ -319,6 +319,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     bool have_pending = false, queue_done = false;
     BatchChunk pend = { 0u, 0u, 0ull };
     uint64_t wave_interactions = 0, wave_truncated = 0, completed = 0, absorbed = 0;
+    uint64_t wave_idle = 0; /* batch: lane-steps spent without a photon while the wave steps */
     uint32_t since_flush = 0;
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
     uint32_t poll_wait = 0; /* serving: steps before re-polling an unpublished ticket */
@@ -390,6 +391,8 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
         const uint64_t n_live = (uint64_t)__popcll(live);
         wave_interactions += n_live;
         absorbed += n_live;
+        if (!SERVE)
+            wave_idle += 64u - n_live;
 
         bool truncated = false;
         if (alive) {
@@ -414,6 +417,8 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
         atomicAdd(a.counters + 1, (unsigned long long)wave_interactions);
         if (wave_truncated)
             atomicAdd(a.counters + 2, (unsigned long long)wave_truncated);
+        if (!SERVE)
+            atomicAdd(a.counters + 4, (unsigned long long)wave_idle);
         /* serving: the last wave to leave rewinds the ticket counter for the
          * next launch (no wave takes a ticket after it has left) */
         if (SERVE && atomicAdd(a.counters + 3, 1ull) == a.total_waves - 1) {
@@ -423,7 +428,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     }
 }
 
-/* Clears the tally pool, the done counters and the queue counters (a
+/* Clears the tally pool, the done counters and the batch counters (a
  * kernel rather than a memset so the sequence can be captured in a graph). */
 __global__ void batch_clear(unsigned long long* tally, uint64_t tally_words, unsigned long long* done,
                             uint32_t nsets, unsigned long long* counters)
@@ -433,7 +438,7 @@ __global__ void batch_clear(unsigned long long* tally, uint64_t tally_words, uns
         tally[i] = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * (uint64_t)nsets; i += stride)
         done[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x < 4)
+    if (blockIdx.x == 0 && threadIdx.x < TMC_BATCH_COUNTERS)
         counters[threadIdx.x] = 0;
 }
 

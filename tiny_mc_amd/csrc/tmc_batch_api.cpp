@@ -43,7 +43,7 @@ struct tmc_batch_plan {
     unsigned long long* d_done = nullptr;
     unsigned long long* d_counters = nullptr;
     unsigned long long* h_tally = nullptr; /* pinned */
-    unsigned long long* h_counters = nullptr; /* pinned, 4 words */
+    unsigned long long* h_counters = nullptr; /* pinned, TMC_BATCH_COUNTERS words */
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     tmc::BatchArgs args;
@@ -90,7 +90,8 @@ hipError_t enqueue_run(tmc_batch_plan* p, hipStream_t stream)
     e = hipMemcpyAsync(p->h_tally, p->d_tally, p->tally_words * sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess)
         return e;
-    return hipMemcpyAsync(p->h_counters, p->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
+    return hipMemcpyAsync(p->h_counters, p->d_counters, TMC_BATCH_COUNTERS * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          stream);
 }
 
 int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, const tmc_batch_options* o,
@@ -220,10 +221,10 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
         || hipMalloc((void**)&p->d_chunks, std::max<size_t>(1, hc.size()) * sizeof(tmc::BatchChunk)) != hipSuccess
         || hipMalloc((void**)&p->d_tally, std::max<uint64_t>(1, p->tally_words) * sizeof(uint64_t)) != hipSuccess
         || hipMalloc((void**)&p->d_done, std::max<size_t>(1, 2 * (size_t)nsets) * sizeof(uint64_t)) != hipSuccess
-        || hipMalloc((void**)&p->d_counters, 4 * sizeof(uint64_t)) != hipSuccess
+        || hipMalloc((void**)&p->d_counters, TMC_BATCH_COUNTERS * sizeof(uint64_t)) != hipSuccess
         || hipHostMalloc((void**)&p->h_tally, std::max<uint64_t>(1, p->tally_words) * sizeof(uint64_t), 0)
             != hipSuccess
-        || hipHostMalloc((void**)&p->h_counters, 4 * sizeof(uint64_t), 0) != hipSuccess) {
+        || hipHostMalloc((void**)&p->h_counters, TMC_BATCH_COUNTERS * sizeof(uint64_t), 0) != hipSuccess) {
         plan_free(p);
         return TMC_E_NOMEM;
     }
@@ -289,6 +290,7 @@ int plan_run_locked(tmc::Context* c, tmc_batch_plan* p, uint64_t* tallies, tmc_b
         st->photons = p->photons;
         st->interactions = p->h_counters[1];
         st->truncated = p->h_counters[2];
+        st->idle_lane_steps = p->h_counters[4];
         st->chunks = p->nchunks;
         st->chunk_work = p->chunk_work;
         st->expected_interactions = p->expected;

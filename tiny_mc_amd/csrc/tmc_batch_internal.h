@@ -59,7 +59,8 @@ struct BatchArgs {
     uint32_t flush_interval; /* wave steps between slot flushes */
     uint32_t ring_mask; /* serve: ring size - 1 (power of two) */
     unsigned long long* counters; /* [0] chunk cursor (serve: tickets of this launch) [1] interactions
-                                     [2] truncated [3] serve: waves that have left */
+                                     [2] truncated [3] serve: waves that have left
+                                     [4] batch: idle lane-steps (TMC_BATCH_COUNTERS words) */
     unsigned long long* tally; /* tally pool */
     unsigned long long* done; /* per set: [2s] photons completed and flushed, [2s+1] absorptions */
     /* serving only */
@@ -73,6 +74,9 @@ struct BatchArgs {
     uint32_t total_waves; /* waves of the launch */
     uint32_t refresh_ticks; /* realtime ticks (100 MHz) between reads of the host control words */
 };
+
+/* 64-bit counter words of a batched launch. */
+#define TMC_BATCH_COUNTERS 8
 
 /* Serving: a set field of a chunk is (slot | generation << 16), so a slot
  * reused by a later request never matches a wave's stale set. */

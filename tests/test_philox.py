@@ -64,3 +64,33 @@ def test_device_philox_kat(tmc, orc):
         assert tuple(int(v) for v in out[i]) == want
     for i in range(0, 4096, 97):
         assert tuple(int(v) for v in out[i]) == tuple(orc.philox(ctr[i].tolist(), key[i].tolist()))
+
+
+def test_block0_shortcut_equals_full_rounds(tmp_path):
+    """tmc_philox4x32_10_block0 (round 1's photon part precomputed) equals the
+    full ten rounds of block (k, 0) on random counters and keys (host build
+    of the same header the kernels include)."""
+    src = tmp_path / "b0.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include "tmc_philox.h"
+static uint64_t s = 88172645463325252ull;
+static uint32_t next(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return (uint32_t)(s >> 32); }
+int main(void) {
+    unsigned bad = 0;
+    for (int i = 0; i < 1000000; ++i) {
+        uint32_t k = next(), c2 = next(), c3 = next(), k0 = next(), k1 = next();
+        if (i & 1) k &= 0xFFFFu;
+        tmc_u32x4 a = tmc_philox4x32_10(k, 0u, c2, c3, k0, k1);
+        tmc_u32x4 b = tmc_philox4x32_10_block0(k, tmc_philox_photon_init(c2, c3, k0, k1), k0, k1);
+        bad += a.x0 != b.x0 || a.x1 != b.x1 || a.x2 != b.x2 || a.x3 != b.x3;
+    }
+    printf("%u\n", bad);
+    return 0;
+}
+''')
+    exe = tmp_path / "b0"
+    inc = os.path.join(ROOT, "tiny_mc_amd", "csrc")
+    subprocess.run(["gcc", "-std=c11", "-O2", "-I", inc, "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    assert out.strip() == "0"

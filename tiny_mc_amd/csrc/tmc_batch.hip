@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
 
     bool alive = false;
     Photon ph;
-    launch_photon(ph, 0);
+    launch_photon(ph, 0, 0u, 0u);
 
     for (;;) {
         uint64_t dead = __ballot(!alive);
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 const uint32_t ndead = (uint32_t)__popcll(dead);
                 const uint32_t rank = mask_rank(dead);
                 if (!alive && rank < avail) {
-                    launch_photon(ph, st.first_photon + pool_next + rank);
+                    launch_photon(ph, st.first_photon + pool_next + rank, st.phys.key0, st.phys.key1);
                     alive = true;
                 }
                 pool_next += avail < ndead ? avail : ndead;

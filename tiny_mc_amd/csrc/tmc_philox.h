@@ -11,8 +11,8 @@
  * 0xCD9E8D57, Weyl key increments 0x9E3779B9 and 0xBB67AE85.
  *
  * On gfx950 each round is two v_mad_u64_u32 (hi and lo of a 32x32 product in
- * one instruction) plus two three-input XORs; the key schedule depends only on
- * the seed, so within a wave it lives in scalar registers.
+ * one instruction) plus XORs; the key schedule depends only on the seed, so
+ * within a wave it lives in scalar registers.
  */
 #ifndef TMC_PHILOX_H
 #define TMC_PHILOX_H
@@ -40,6 +40,49 @@ TMC_HD tmc_u32x4 tmc_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32
 #pragma unroll
 #endif
     for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = (uint64_t)c0 * TMC_PHILOX_M0;
+        const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += TMC_PHILOX_W0;
+        k1 += TMC_PHILOX_W1;
+    }
+    tmc_u32x4 r = { c0, c1, c2, c3 };
+    return r;
+}
+
+/* The part of round 1 of block (k, 0) that depends only on the photon
+ * (counter words 2-3) and the key: for counter (k, 0, c2, c3) round 1 gives
+ *   c0' = hi(c2 * M1) ^ 0 ^ key0,  c1' = lo(c2 * M1),
+ *   c2' = hi(k * M0) ^ c3 ^ key1,  c3' = lo(k * M0),
+ * so a walk computes c0', c1' and c3 ^ key1 once per photon and only the
+ * product k * M0 per absorption. */
+typedef struct tmc_philox_photon {
+    uint32_t c0r1, c1r1, c3k1;
+} tmc_philox_photon;
+
+TMC_HD tmc_philox_photon tmc_philox_photon_init(uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
+{
+    const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
+    tmc_philox_photon pp = { (uint32_t)(p1 >> 32) ^ k0, (uint32_t)p1, c3 ^ k1 };
+    return pp;
+}
+
+/* tmc_philox4x32_10(k, 0, c2, c3, k0, k1) from the photon's precomputed part. */
+TMC_HD tmc_u32x4 tmc_philox4x32_10_block0(uint32_t k, tmc_philox_photon pp, uint32_t k0, uint32_t k1)
+{
+    const uint64_t q0 = (uint64_t)k * TMC_PHILOX_M0;
+    uint32_t c0 = pp.c0r1, c1 = pp.c1r1, c2 = (uint32_t)(q0 >> 32) ^ pp.c3k1, c3 = (uint32_t)q0;
+    k0 += TMC_PHILOX_W0;
+    k1 += TMC_PHILOX_W1;
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int round = 1; round < 10; ++round) {
         const uint64_t p0 = (uint64_t)c0 * TMC_PHILOX_M0;
         const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
         const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;

@@ -17,6 +17,7 @@ struct Photon {
     float x, y, z, u, v, w, weight;
     uint32_t k; /* absorptions so far = Philox counter word 0 */
     uint32_t plo, phi; /* photon index = Philox counter words 2-3 */
+    tmc_philox_photon pp; /* the photon's part of Philox round 1 */
 };
 
 /* Physics of the photon's parameter set. */
@@ -28,7 +29,7 @@ struct Physics {
     uint32_t key0, key1; /* Philox key = 64-bit seed */
 };
 
-__device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon)
+__device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon, uint32_t key0, uint32_t key1)
 {
     p.x = p.y = p.z = 0.0f;
     p.u = p.v = 0.0f;
@@ -37,6 +38,7 @@ __device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon)
     p.k = 0;
     p.plo = (uint32_t)photon;
     p.phi = (uint32_t)(photon >> 32);
+    p.pp = tmc_philox_photon_init(p.plo, p.phi, key0, key1);
 }
 
 /* LDS words one lane deposits into: the shell sums (w and w*w images) and
@@ -56,7 +58,7 @@ struct LaneTally {
 template <uint32_t SAMPLER>
 __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt)
 {
-    const tmc_u32x4 r = tmc_philox4x32_10(p.k, 0u, p.plo, p.phi, ph.key0, ph.key1);
+    const tmc_u32x4 r = tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1); /* block (k, 0) */
 
     /* move (photon.c:21-24) */
     const float t = -log32_u24(tmc_u24(r.x0)); /* = -log32(tmc_u01(r.x0)) */

@@ -50,6 +50,7 @@ struct PhotonW {
     uint32_t r; /* roulette survivals */
     uint32_t k; /* absorptions so far = Philox counter word 0 */
     uint32_t plo, phi;
+    tmc_philox_photon pp;
 };
 
 struct PhysicsW {
@@ -61,7 +62,7 @@ struct PhysicsW {
     uint32_t key0, key1;
 };
 
-__device__ __forceinline__ void launch_photon(PhotonW& p, uint64_t photon)
+__device__ __forceinline__ void launch_photon(PhotonW& p, uint64_t photon, uint32_t key0, uint32_t key1)
 {
     p.x = p.y = p.z = 0.0f;
     p.xl = p.yl = p.zl = 0.0f;
@@ -73,6 +74,7 @@ __device__ __forceinline__ void launch_photon(PhotonW& p, uint64_t photon)
     p.k = 0;
     p.plo = (uint32_t)photon;
     p.phi = (uint32_t)(photon >> 32);
+    p.pp = tmc_philox_photon_init(p.plo, p.phi, key0, key1);
 }
 
 /* 2^(e * (ahi + alo) + r * (shi + slo)) to within about one float rounding
@@ -109,7 +111,7 @@ __device__ __forceinline__ void compensated_move(float& x, float& xl, float t, f
 template <uint32_t SAMPLER, bool COMP>
 __device__ __forceinline__ bool step(PhotonW& p, const PhysicsW& ph, const LaneTally& lt)
 {
-    const tmc_u32x4 r = tmc_philox4x32_10(p.k, 0u, p.plo, p.phi, ph.key0, ph.key1);
+    const tmc_u32x4 r = tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1); /* block (k, 0) */
 
     /* move (photon.c:21-24) */
     const float t = -log32_u24(tmc_u24(r.x0)); /* = -log32(tmc_u01(r.x0)) */
@@ -187,6 +189,7 @@ struct Photon64 {
     double x, y, z, u, v, w, weight;
     uint32_t k;
     uint32_t plo, phi;
+    tmc_philox_photon pp;
 };
 
 struct Physics64 {
@@ -197,7 +200,7 @@ struct Physics64 {
     uint32_t key0, key1;
 };
 
-__device__ __forceinline__ void launch_photon(Photon64& p, uint64_t photon)
+__device__ __forceinline__ void launch_photon(Photon64& p, uint64_t photon, uint32_t key0, uint32_t key1)
 {
     p.x = p.y = p.z = 0.0;
     p.u = p.v = 0.0;
@@ -206,6 +209,7 @@ __device__ __forceinline__ void launch_photon(Photon64& p, uint64_t photon)
     p.k = 0;
     p.plo = (uint32_t)photon;
     p.phi = (uint32_t)(photon >> 32);
+    p.pp = tmc_philox_photon_init(p.plo, p.phi, key0, key1);
 }
 
 __device__ __forceinline__ uint64_t fx_heat64(double w) { return (uint64_t)__double2ull_rn(w * 0x1p33); }
@@ -214,7 +218,7 @@ __device__ __forceinline__ uint64_t fx_heat2_64(double w) { return (uint64_t)__d
 template <uint32_t SAMPLER>
 __device__ __forceinline__ bool step(Photon64& p, const Physics64& ph, const LaneTally& lt)
 {
-    const tmc_u32x4 r = tmc_philox4x32_10(p.k, 0u, p.plo, p.phi, ph.key0, ph.key1);
+    const tmc_u32x4 r = tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1); /* block (k, 0) */
 
     /* move (photon.c:21-24) */
     const double t = -log((double)tmc_u01(r.x0));

@@ -112,7 +112,7 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
     /* lane state: one photon (photon.c:11-18) */
     bool alive = false;
     PhotonT ph;
-    launch_photon(ph, 0);
+    launch_photon(ph, 0, phys.key0, phys.key1);
 
     for (;;) {
         const uint64_t dead = __ballot(!alive);
@@ -144,7 +144,7 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
                     got = true;
                 }
                 if (got) {
-                    launch_photon(ph, a.first_photon + idx);
+                    launch_photon(ph, a.first_photon + idx, phys.key0, phys.key1);
                     alive = true;
                 }
             }

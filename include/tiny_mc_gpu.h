@@ -43,7 +43,7 @@ typedef enum tmc_status {
     TMC_E_HIP = -3, /* a HIP runtime call failed */
     TMC_E_NOMEM = -4, /* device or host allocation failed */
     TMC_E_RANGE = -5, /* photon index range overflows 64 bits */
-    TMC_E_UNSUPPORTED = -6 /* option not supported by this build */
+    TMC_E_UNSUPPORTED = -6 /* option or library build not available */
 } tmc_status;
 
 /* Physics and size of one simulation; the fields are the macros of

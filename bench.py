@@ -209,6 +209,27 @@ def serve_leg(capacity_interactions_per_s, load=0.5, seconds=15.0, check=16):
     }
 
 
+def reference_case_leg(sampler, calls=20):
+    """The reference's own configuration (params.h defaults: 32,768 photons,
+    101 shells): too small to fill the GPU, so the figure is the latency of
+    one synchronous call (median of `calls`)."""
+    import tiny_mc_amd
+
+    p = tiny_mc_amd.Params(photons=32768)
+    total, kern = [], []
+    for i in range(calls + 2):
+        _, st = tiny_mc_amd.run_raw(p, 12345 + i, sampler=sampler)
+        if i >= 2:
+            total.append(st.total_ms)
+            kern.append(st.kernel_ms)
+    return {
+        "photons": p.photons,
+        "call_ms_p50": float(sorted(total)[len(total) // 2]),
+        "kernel_ms_p50": float(sorted(kern)[len(kern) // 2]),
+        "photons_per_s_per_call": p.photons / (sorted(total)[len(total) // 2] * 1e-3),
+    }
+
+
 def read_traffic():
     """HBM bytes per walk launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -310,6 +331,7 @@ def main():
     }
     if result["cpu_baseline"]:
         progress(f"cpu baseline: {result['cpu_baseline']['value']:.4g} photons/s")
+    result["reference_case"] = reference_case_leg(sampler)
     if not args.no_sweep:
         result["sweep"] = sweep_leg()
         progress(f"sweep: {result['sweep']['photons_per_s']:.4g} photons/s, "

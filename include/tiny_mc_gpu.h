@@ -154,7 +154,7 @@ int tiny_mc_gpu_check_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismat
  * No counterpart in the reference (it has one compile-time parameter set);
  * this is the loop of tiny_mc.c:47-49 run for many params.h configurations
  * at once.  Each set's tallies are bit-identical to tiny_mc_gpu_run_ex on
- * that set alone.
+ * that set alone (binary32 arithmetic, TMC_PRECISION_FP32).
  * ------------------------------------------------------------------------ */
 
 /* One simulation of a batch: physics and photon count, its seed, and the
@@ -226,12 +226,14 @@ int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc_batch_opti
  * Serving: a request loop over a resident persistent kernel.
  *
  * No counterpart in the reference.  Requests (one tmc_set each) are admitted
- * while tally arena, request slots and chunk ring have room, in arrival
- * order; their chunks are published round-robin into a ring in
- * host-coherent memory that the running kernel polls, so requests that
- * arrive while others are on the GPU join the same launch.  Every result is
- * bit-identical to tiny_mc_gpu_run_ex of the same set alone.  A server owns
- * a host thread and two default-priority streams on the context's device.
+ * in arrival order while a request slot and room in the tally arena are
+ * free; their chunks are published, shortest remaining work first, into a
+ * ring in host-coherent memory that the running kernel polls, so requests
+ * that arrive while others are on the GPU join the same launch.  Every
+ * result is bit-identical to tiny_mc_gpu_run_ex of the same set alone
+ * (binary32 arithmetic, TMC_PRECISION_FP32).  A server owns a host thread
+ * and one default-priority stream on the context's device; while it holds
+ * requests its kernel occupies all but a few workgroup slots of the device.
  * ------------------------------------------------------------------------ */
 
 #define TMC_PENDING 1 /* positive status: the request has not finished */

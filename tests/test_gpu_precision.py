@@ -97,3 +97,9 @@ def test_default_physics_within_fp32_tolerance(tmc, orc, precision):
     bad = np.nonzero(np.abs(heat - oh) > tol)[0]
     assert bad.size == 0, (bad, heat[bad], oh[bad], tol[bad])
     assert st.interactions == on
+
+
+def test_invalid_precision_rejected(tmc):
+    with pytest.raises(tmc.TmcError) as e:
+        tmc.run_raw(tmc.Params(photons=64), 1, precision=4)
+    assert e.value.status == -1

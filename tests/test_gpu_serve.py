@@ -121,6 +121,24 @@ def test_single_runs_while_serving(tmc):
     np.testing.assert_array_equal(a[0], alone(tmc, *long_req))
 
 
+def test_two_servers_share_the_device(tmc):
+    """Two servers, each with its resident kernel, on one device at once:
+    neither waits on the other (no copies while a kernel runs), every result
+    is exact, and a closed server refuses new requests."""
+    reqs = mixed_requests(tmc, 24, seed=41)
+    a, b = tmc.Server(), tmc.Server(max_shells=401)
+    try:
+        ids = [(srv, srv.submit(p, s)) for i, (p, s) in enumerate(reqs) for srv in [(a, b)[i % 2]]]
+        got = [srv.wait(rid) for srv, rid in ids]
+    finally:
+        a.close()
+        b.close()
+    for (p, s), (t, _) in zip(reqs, got):
+        np.testing.assert_array_equal(t, alone(tmc, p, s))
+    with pytest.raises(tmc.TmcError):
+        a.submit(tmc.Params(), 1)
+
+
 def test_close_finishes_submitted(tmc):
     srv = tmc.Server()
     for i in range(5):

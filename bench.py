@@ -255,6 +255,8 @@ def main():
     ap.add_argument("--rocpd-summary", metavar="DB", help="print a text summary of a rocprofv3 results database")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-reference-case", action="store_true",
+                    help="skip the 32,768-photon call latency (keeps profiled runs to the 2^30-photon launches)")
     ap.add_argument("--serve-load", type=float, default=0.5)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -331,7 +333,8 @@ def main():
     }
     if result["cpu_baseline"]:
         progress(f"cpu baseline: {result['cpu_baseline']['value']:.4g} photons/s")
-    result["reference_case"] = reference_case_leg(sampler)
+    if not args.no_reference_case:
+        result["reference_case"] = reference_case_leg(sampler)
     if not args.no_sweep:
         result["sweep"] = sweep_leg()
         progress(f"sweep: {result['sweep']['photons_per_s']:.4g} photons/s, "

@@ -30,7 +30,7 @@ def mean_request_work(requests):
     return float(np.mean([p.photons * expected_interactions(p) for p, _ in requests]))
 
 
-def run_open_loop(server, requests, rate_per_s, seed=1, progress=None):
+def run_open_loop(server, requests, rate_per_s, seed=1, progress=None, timeout_s=300.0):
     """Submits `requests` at Poisson arrival times with `rate_per_s` and
     waits for every result.  Returns per-request records (dict) and the wall
     time from the first arrival to the last completion."""
@@ -55,7 +55,10 @@ def run_open_loop(server, requests, rate_per_s, seed=1, progress=None):
     for i in range(len(requests)):
         while ids[i] is None:
             time.sleep(1e-4)
-        tally, st = server.wait(ids[i])
+        res = server.wait(ids[i], timeout=timeout_s)
+        if res is None:
+            raise TimeoutError(f"request {i} not finished after {timeout_s} s: {server.stats()}")
+        tally, st = res
         records[i] = {"tally": tally, "stats": st}
         if progress and (i + 1) % 2000 == 0:
             progress(f"serve: {i + 1}/{len(requests)} requests done, {time.perf_counter() - t0:.1f} s")

@@ -69,6 +69,15 @@ typedef struct tmc_params {
 #define TMC_PRECISION_FP32_COMPENSATED 2u /* as _WEIGHT, plus compensated position sums */
 #define TMC_PRECISION_FP64 3u /* every quantity in binary64 */
 
+/* How the binary32 walk evaluates its two transcendental quantities, the
+ * step length -ln u (photon.c:21) and the direction's sin/cos of 2 pi u'.
+ * Both are accurate to about one unit in the last place (the hardware log:
+ * <= 1.92 ulp on every input the walk can produce; sin/cos: <= 1.3e-7
+ * absolute); the own polynomials exist so that a host program can
+ * reproduce the walk bit for bit without evaluating anything on the device. */
+#define TMC_MATH_HARDWARE 0u /* v_log_f32, v_sin_f32, v_cos_f32 (default) */
+#define TMC_MATH_OWN 1u /* own minimax polynomials (tmc_math.h) */
+
 typedef struct tmc_options {
     uint32_t sampler; /* TMC_SAMPLER_* */
     uint32_t block; /* threads per workgroup, multiple of 64 in [64, 1024]; 0 = auto */
@@ -77,7 +86,8 @@ typedef struct tmc_options {
     uint64_t first_photon; /* index of the first photon of the stream to walk */
     uint64_t max_interactions; /* stop a photon after this many absorptions; 0 = never */
     uint32_t precision; /* TMC_PRECISION_* */
-    uint32_t reserved; /* 0 */
+    uint32_t math; /* TMC_MATH_* (binary32 walk of TMC_PRECISION_FP32; the other
+                      precision modes use the own polynomials or binary64) */
 } tmc_options;
 
 typedef struct tmc_run_stats {
@@ -89,7 +99,7 @@ typedef struct tmc_run_stats {
     uint32_t grid, block, chunk; /* launch shape used */
     uint32_t sampler;
     uint32_t precision;
-    uint32_t reserved;
+    uint32_t math;
 } tmc_run_stats;
 
 /* Default options (analytic sampler, automatic launch shape, photon 0). */
@@ -142,6 +152,11 @@ const char* tiny_mc_gpu_strerror(int status);
 int tiny_mc_gpu_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out);
 int tiny_mc_gpu_eval_math(const float* in, uint32_t n, float* log_out, float* sin_out, float* cos_out);
 
+/* The walk's own step length -ln(u) and sin/cos(2 pi u) in math mode `math`
+ * (TMC_MATH_*) on n inputs: the exact values a binary32 walk uses. */
+int tiny_mc_gpu_eval_step_math(uint32_t math, const float* in, uint32_t n, float* step_out, float* sin_out,
+                               float* cos_out);
+
 /* Diagnostic: compares the walk's short correctly rounded square root with
  * the device's IEEE square root on every float whose bit pattern lies in
  * [first_bits, first_bits + count); returns the number of mismatches and
@@ -173,7 +188,7 @@ typedef struct tmc_batch_options {
     uint64_t chunk_work; /* target absorptions per queue chunk; 0 = auto */
     uint64_t max_interactions; /* per photon, 0 = never stop early */
     uint32_t use_graph; /* nonzero: plans replay a captured HIP graph */
-    uint32_t reserved;
+    uint32_t math; /* TMC_MATH_* */
 } tmc_batch_options;
 
 typedef struct tmc_batch_stats {
@@ -248,6 +263,8 @@ typedef struct tmc_serve_options {
     uint64_t chunk_work; /* target absorptions per chunk; 0 = 2^17 */
     uint32_t max_photons_per_lane; /* cap on photons per lane in a chunk; 0 = 16 */
     uint32_t idle_ms; /* the server stops the kernel after this long without requests; 0 = 50 */
+    uint32_t math; /* TMC_MATH_* */
+    uint32_t reserved; /* 0 */
 } tmc_serve_options;
 
 typedef struct tmc_request_stats {

@@ -23,13 +23,18 @@
  *     (weight cadence, mean interactions, energy balance, first-step shell
  *     probability, direction moments) in tests/test_oracle_*.py.
  *
- *  3. orc_mirror32
+ *  3. orc_mirror32 / orc_mirror32_tab
  *     The GPU kernel's exact FP32 arithmetic (same Philox words, same own log
  *     and sin/cos polynomials, fused multiply-adds where the kernel fuses,
  *     correctly rounded sqrt and division) with the kernel's 64-bit
  *     fixed-point tallies, so GPU tallies can be compared bit for bit.  Pinned
  *     against orc_walk64 per photon and per shell, and its log/sin/cos against
- *     libm over every possible input.
+ *     libm over every possible input.  For the kernel's hardware-math mode
+ *     (v_log_f32, v_sin_f32, v_cos_f32, whose exact rounding is not
+ *     published), orc_mirror32_tab takes the step length and sin/cos for each
+ *     of the 2^24 uniforms the walk can draw as tables (evaluated by the
+ *     device; their accuracy is pinned against libm separately) and restates
+ *     every other operation of the walk as above.
  *
  * Build: oracle/Makefile (gcc -std=c11 -ffp-contract=off; no FP contraction,
  * matching the reference's ISO-C build, Makefile:5).
@@ -434,6 +439,8 @@ typedef struct m32_params {
     uint32_t shells;
     int sampler;
     uint64_t max_k;
+    /* NULL: own log / sin / cos; else per uniform index r >> 8 */
+    const float *t_tab, *s_tab, *c_tab;
 } m32_params;
 
 /* Fixed-point images of one deposit: w * 2^33 and (w*w) * 2^43, truncated.
@@ -454,7 +461,7 @@ static uint64_t mirror_photon(uint64_t seed, uint64_t photon, const m32_params* 
     for (;;) {
         uint32_t b0[4];
         orc_block(seed, photon, k, 0, b0);
-        const float t = -orc_log32(orc_u01(b0[0]));
+        const float t = mp->t_tab ? mp->t_tab[b0[0] >> 8] : -orc_log32(orc_u01(b0[0]));
         x = fmaf(t, u, x);
         y = fmaf(t, v, y);
         z = fmaf(t, w, z);
@@ -495,7 +502,12 @@ static uint64_t mirror_photon(uint64_t seed, uint64_t photon, const m32_params* 
             const float mu = fmaf(2.0f, orc_u01(b0[1]), -1.0f);
             const float s = sqrtf(fmaf(-mu, mu, 1.0f));
             float sn, cs;
-            orc_sincos2pi32(orc_u01(b0[2]), &sn, &cs);
+            if (mp->s_tab) {
+                sn = mp->s_tab[b0[2] >> 8];
+                cs = mp->c_tab[b0[2] >> 8];
+            } else {
+                orc_sincos2pi32(orc_u01(b0[2]), &sn, &cs);
+            }
             u = mu;
             v = s * cs;
             w = s * sn;
@@ -540,12 +552,14 @@ ORC_EXPORT void orc_derived32(const orc_params* p, float* albedo, float* spm)
 /* FP32 mirror over photons [first, first + count).  tally holds 4*shells
  * 64-bit words: (lo, hi) of the 128-bit sum of w*2^33 for each shell, then
  * (lo, hi) of the 128-bit sum of (w*w)*2^43 for each shell. */
-ORC_EXPORT int orc_mirror32(const orc_params* p, uint64_t seed, uint64_t first, uint64_t count, int sampler,
-                            uint64_t max_k, int threads, uint64_t* tally, uint64_t* interactions)
+ORC_EXPORT int orc_mirror32_tab(const orc_params* p, uint64_t seed, uint64_t first, uint64_t count, int sampler,
+                                uint64_t max_k, int threads, const float* t_tab, const float* s_tab,
+                                const float* c_tab, uint64_t* tally, uint64_t* interactions)
 {
-    if (!orc_params_ok(p) || !tally || (sampler != ORC_SAMPLER_ANALYTIC && sampler != ORC_SAMPLER_REJECTION))
+    if (!orc_params_ok(p) || !tally || (sampler != ORC_SAMPLER_ANALYTIC && sampler != ORC_SAMPLER_REJECTION)
+        || (t_tab && (!s_tab || !c_tab)))
         return -1;
-    m32_params mp = { restate_albedo(p), restate_spm(p), p->shells, sampler, max_k };
+    m32_params mp = { restate_albedo(p), restate_spm(p), p->shells, sampler, max_k, t_tab, s_tab, c_tab };
     threads = clamp_threads(threads, count);
     const size_t S = p->shells;
     u128* acc = calloc((size_t)threads * 2 * S, sizeof(u128));
@@ -581,6 +595,13 @@ ORC_EXPORT int orc_mirror32(const orc_params* p, uint64_t seed, uint64_t first, 
     if (interactions)
         *interactions = n;
     return 0;
+}
+
+/* orc_mirror32_tab with the own log / sin / cos polynomials. */
+ORC_EXPORT int orc_mirror32(const orc_params* p, uint64_t seed, uint64_t first, uint64_t count, int sampler,
+                            uint64_t max_k, int threads, uint64_t* tally, uint64_t* interactions)
+{
+    return orc_mirror32_tab(p, seed, first, count, sampler, max_k, threads, NULL, NULL, NULL, tally, interactions);
 }
 
 /* ------------------------------------------------------------------------ */

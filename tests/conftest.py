@@ -29,3 +29,10 @@ def tmc():
 
     tiny_mc_amd.load()
     return tiny_mc_amd
+
+
+@pytest.fixture(scope="session")
+def hw_tables(tmc):
+    """The device's step length and sin/cos for every uniform the walk can
+    draw in the hardware-math mode (indexed by r >> 8), for the mirror."""
+    return tmc.step_math_tables(tmc.MATH_HARDWARE)

@@ -78,7 +78,7 @@ def test_strerror_and_defaults(tmc):
     o = _Options(7, 7, 7, 7, 7, 7, 7, 7)
     lib.tmc_options_default(ctypes.byref(o))
     assert (o.sampler, o.block, o.grid, o.chunk, o.first_photon, o.max_interactions, o.precision,
-            o.reserved) == (0, 0, 0, 0, 0, 0, 0, 0)
+            o.math) == (0, 0, 0, 0, 0, 0, 0, 0)
     assert ctypes.sizeof(o) == 40
 
 

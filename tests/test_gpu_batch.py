@@ -34,12 +34,15 @@ def test_batch_equals_sets_alone(tmc, sampler):
     assert 0.0 < st.lane_utilisation <= 1.0
 
 
-def test_batch_equals_mirror(tmc, orc):
+@pytest.mark.parametrize("math", [0, 1])
+def test_batch_equals_mirror(tmc, orc, hw_tables, math):
     sets = small_sweep(tmc, n=6, scale=1.0 / 1024, seed=9)
-    flat, st = tmc.batch_run(sets)
+    flat, st = tmc.batch_run(sets, tmc.BatchOptions(math=math))
     total = 0
+    tables = hw_tables if math == tmc.MATH_HARDWARE else None
     for (p, seed), t in zip(sets, tmc.split_tallies(sets, flat)):
-        want, n = orc.mirror32(orc.params(p.mu_a, p.mu_s, p.shells, p.microns_per_shell), seed, p.photons)
+        want, n = orc.mirror32(orc.params(p.mu_a, p.mu_s, p.shells, p.microns_per_shell), seed, p.photons,
+                               tables=tables)
         np.testing.assert_array_equal(t, want)
         total += n
     assert st.interactions == total

@@ -31,7 +31,7 @@ def test_driver_table_matches_mirror(tmc, orc):
     assert len(rows) == 100 and lines[-1].startswith("# extra\t")
     got = np.array([[float(v) for v in r] for r in rows])
 
-    tally, _ = orc.mirror32(orc.params(), 12345, 65536)
+    tally, _ = orc.mirror32(orc.params(), 12345, 65536, tables=tmc.step_math_tables(tmc.MATH_HARDWARE))
     heat, heat2 = oracle_lib.tally_heat(tally, 101, 2.0, 20.0)
     rows_want, extra = tmc.table_rows(heat, heat2, 65536, 50.0)
     want = np.array(rows_want)

@@ -2,9 +2,13 @@
 
 Bit-exact: the kernel's 128-bit integer tallies equal the FP32 mirror's
 (oracle.c orc_mirror32) on the same Philox stream, for both direction
-samplers and across physics, shell counts, photon-index offsets and launch
-shapes.  Tolerance: at 2^24 photons every shell's heat is within
-max(3 sigma from heat2, 1e-5 relative) of the FP64 oracle (orc_walk64)."""
+samplers, both math modes and across physics, shell counts, photon-index
+offsets and launch shapes.  In the hardware-math mode (the default) the
+mirror takes the device's step length and sin/cos for each of the 2^24
+uniforms the walk can draw (their accuracy against float64 is checked here
+on every one of them) and restates every other operation.  Tolerance: at
+2^24 photons every shell's heat is within max(3 sigma from heat2, 1e-5
+relative) of the FP64 oracle (orc_walk64)."""
 import numpy as np
 import pytest
 
@@ -25,6 +29,7 @@ def _mirror(orc, phys, seed, photons, **kw):
     return orc.mirror32(orc.params(**phys), seed, photons, **kw)
 
 
+@pytest.mark.parametrize("math", [0, 1])
 @pytest.mark.parametrize("sampler", [0, 1])
 @pytest.mark.parametrize("phys,seed,photons,first", [
     (DEFAULT, 12345, 1 << 16, 0),
@@ -35,28 +40,30 @@ def _mirror(orc, phys, seed, photons, **kw):
     (dict(DEFAULT, shells=2), 9, 5000, 0),
     (dict(DEFAULT, shells=401, microns_per_shell=10.0), 77, 10000, 0),
 ])
-def test_tallies_bit_exact_vs_mirror(tmc, orc, sampler, phys, seed, photons, first):
-    tally, st = tmc.run_raw(_params(tmc, phys, photons), seed, sampler=sampler, first_photon=first)
-    want, n = _mirror(orc, phys, seed, photons, first=first, sampler=sampler)
+def test_tallies_bit_exact_vs_mirror(tmc, orc, hw_tables, math, sampler, phys, seed, photons, first):
+    tally, st = tmc.run_raw(_params(tmc, phys, photons), seed, sampler=sampler, first_photon=first, math=math)
+    assert st.math == math
+    tables = hw_tables if math == tmc.MATH_HARDWARE else None
+    want, n = _mirror(orc, phys, seed, photons, first=first, sampler=sampler, tables=tables)
     assert st.interactions == n
     assert st.photons == photons and st.truncated == 0
     np.testing.assert_array_equal(tally, want)
 
 
 @pytest.mark.parametrize("block,grid,chunk", [(64, 1, 64), (256, 7, 128), (1024, 0, 4096), (512, 3000, 64)])
-def test_launch_shape_independence(tmc, orc, block, grid, chunk):
+def test_launch_shape_independence(tmc, orc, hw_tables, block, grid, chunk):
     p = _params(tmc, DEFAULT, 50000)
-    want, n = _mirror(orc, DEFAULT, 4242, 50000)
+    want, n = _mirror(orc, DEFAULT, 4242, 50000, tables=hw_tables)
     tally, st = tmc.run_raw(p, 4242, block=block, grid=grid, chunk=chunk)
     np.testing.assert_array_equal(tally, want)
     assert st.interactions == n
 
 
-def test_truncation_matches_mirror(tmc, orc):
+def test_truncation_matches_mirror(tmc, orc, hw_tables):
     p = _params(tmc, DEFAULT, 30000)
     for kmax in (1, 2, 50):
         tally, st = tmc.run_raw(p, 5, max_interactions=kmax)
-        want, n = _mirror(orc, DEFAULT, 5, 30000, max_k=kmax)
+        want, n = _mirror(orc, DEFAULT, 5, 30000, max_k=kmax, tables=hw_tables)
         np.testing.assert_array_equal(tally, want)
         assert st.interactions == n
         if kmax == 1:
@@ -139,3 +146,26 @@ def test_short_sqrt_exhaustive(tmc):
     hi = int(np.float32(np.finfo(np.float32).max).view(np.uint32))
     assert tmc.check_sqrt(0, 1) == (0, 0)
     assert tmc.check_sqrt(lo, hi - lo + 1) == (0, 0)
+
+
+def test_hardware_math_accuracy_exhaustive(tmc, orc, hw_tables):
+    """Hardware-math step length -ln(u) within 2 ulp and sin/cos(2 pi u)
+    within 2e-7 of float64 on every uniform the walk can draw; the own-math
+    evaluation equals the host polynomials bit for bit."""
+    t, sn, cs = hw_tables
+    m = np.arange(1, (1 << 24) + 1, dtype=np.float64)
+    u = m * 2.0 ** -24
+    ref = -np.log(u)
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    ulp[ulp == 0] = np.spacing(np.float32(0))
+    assert np.max(np.abs(t - ref) / ulp) <= 2.0
+    assert np.max(np.abs(sn - np.sin(2 * np.pi * u))) <= 2e-7
+    assert np.max(np.abs(cs - np.cos(2 * np.pi * u))) <= 2e-7
+    # the own polynomials as the walk evaluates them, against the host's
+    idx = np.arange(0, 1 << 24, 97)
+    uf = u[idx].astype(np.float32)
+    ot, osn, ocs = tmc.eval_step_math(uf, tmc.MATH_OWN)
+    lg, hs, hc = orc.eval_math32(uf)
+    np.testing.assert_array_equal(ot.view(np.uint32), (-lg).view(np.uint32))
+    np.testing.assert_array_equal(osn.view(np.uint32), hs.view(np.uint32))
+    np.testing.assert_array_equal(ocs.view(np.uint32), hc.view(np.uint32))

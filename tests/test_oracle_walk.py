@@ -143,3 +143,21 @@ def test_derived_constants_match_reference_arithmetic(orc):
     a, spm = orc.derived32(orc.params(**DEFAULT))
     assert a == np.float32(np.float32(20.0) / np.float32(22.0))
     assert spm == np.float32(1e4 / 50.0 / float(np.float32(22.0)))
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_mirror_table_mode_equals_own_math(orc, sampler):
+    """The mirror's table mode (used for the kernel's hardware math) looks up
+    entry r >> 8: fed the own polynomials' values it reproduces the
+    own-math mirror bit for bit."""
+    u = ((np.arange(1 << 24, dtype=np.float64) + 1.0) * 2.0 ** -24).astype(np.float32)
+    lg, sn, cs = orc.eval_math32(u)
+    tables = (-lg, sn, cs)
+    p = orc.params()
+    a, na = orc.mirror32(p, 77, 4096, sampler=sampler)
+    b, nb = orc.mirror32(p, 77, 4096, sampler=sampler, tables=tables)
+    np.testing.assert_array_equal(a, b)
+    assert na == nb
+    # a table that differs from the polynomials changes the tallies
+    c, _ = orc.mirror32(p, 77, 4096, sampler=sampler, tables=(-lg * np.float32(1.001), sn, cs))
+    assert not np.array_equal(a, c)

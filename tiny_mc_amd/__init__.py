@@ -19,6 +19,8 @@ from ._lib import (  # noqa: F401
     split_tallies,
     HEAT2_FRAC_BITS,
     HEAT_FRAC_BITS,
+    MATH_HARDWARE,
+    MATH_OWN,
     MAX_SHELLS,
     PRECISION_FP32,
     PRECISION_FP32_COMPENSATED,
@@ -32,6 +34,8 @@ from ._lib import (  # noqa: F401
     check_sqrt,
     device_info,
     eval_math,
+    eval_step_math,
+    step_math_tables,
     lib_path,
     load,
     philox,

@@ -20,6 +20,8 @@ PRECISION_FP32 = 0
 PRECISION_FP32_WEIGHT = 1
 PRECISION_FP32_COMPENSATED = 2
 PRECISION_FP64 = 3
+MATH_HARDWARE = 0
+MATH_OWN = 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 lib_path = os.path.join(_HERE, "libtiny_mc_gpu.so")
@@ -62,7 +64,7 @@ class _Options(ctypes.Structure):
         ("first_photon", ctypes.c_uint64),
         ("max_interactions", ctypes.c_uint64),
         ("precision", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("math", ctypes.c_uint32),
     ]
 
 
@@ -78,7 +80,7 @@ class _Stats(ctypes.Structure):
         ("chunk", ctypes.c_uint32),
         ("sampler", ctypes.c_uint32),
         ("precision", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("math", ctypes.c_uint32),
     ]
 
 
@@ -108,6 +110,7 @@ class RunStats:
     chunk: int
     sampler: int
     precision: int
+    math: int
 
 
 _lib = None
@@ -140,8 +143,10 @@ def load():
     lib.tiny_mc_gpu_philox.argtypes = [u32p, u32p, ctypes.c_uint32, u32p]
     lib.tiny_mc_gpu_eval_math.argtypes = [fp, ctypes.c_uint32, fp, fp, fp]
     lib.tiny_mc_gpu_check_sqrt.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u64p, P(ctypes.c_uint32)]
+    lib.tiny_mc_gpu_eval_step_math.argtypes = [ctypes.c_uint32, fp, ctypes.c_uint32, fp, fp, fp]
     for name in ("tiny_mc_gpu_run", "tiny_mc_gpu_run_ex", "tiny_mc_gpu_tally_to_heat", "tiny_mc_gpu_init",
-                 "tiny_mc_gpu_device_info", "tiny_mc_gpu_philox", "tiny_mc_gpu_eval_math", "tiny_mc_gpu_check_sqrt"):
+                 "tiny_mc_gpu_device_info", "tiny_mc_gpu_philox", "tiny_mc_gpu_eval_math", "tiny_mc_gpu_check_sqrt",
+                 "tiny_mc_gpu_eval_step_math"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -168,17 +173,17 @@ def run(params: Params, seed: int):
 
 
 def run_raw(params: Params, seed: int, *, sampler=SAMPLER_ANALYTIC, first_photon=0, max_interactions=0,
-            block=0, grid=0, chunk=0, precision=PRECISION_FP32):
+            block=0, grid=0, chunk=0, precision=PRECISION_FP32, math=MATH_HARDWARE):
     """Integer tallies (uint64 array of 4*shells: 128-bit (lo, hi) sums) and RunStats."""
     lib = load()
     tally = np.zeros(4 * params.shells, dtype=np.uint64)
-    opt = _Options(sampler, block, grid, chunk, first_photon, max_interactions, precision, 0)
+    opt = _Options(sampler, block, grid, chunk, first_photon, max_interactions, precision, math)
     st = _Stats()
     c = params._c()
     _check(lib.tiny_mc_gpu_run_ex(ctypes.byref(c), seed, ctypes.byref(opt), _ptr(tally, ctypes.c_uint64),
                                   ctypes.byref(st)), "tiny_mc_gpu_run_ex")
     stats = RunStats(st.photons, st.interactions, st.truncated, st.kernel_ms, st.total_ms, st.grid, st.block,
-                     st.chunk, st.sampler, st.precision)
+                     st.chunk, st.sampler, st.precision, st.math)
     return tally, stats
 
 
@@ -205,6 +210,25 @@ def philox(ctr, key):
     _check(lib.tiny_mc_gpu_philox(_ptr(ctr, ctypes.c_uint32), _ptr(key, ctypes.c_uint32), ctr.shape[0],
                                   _ptr(out, ctypes.c_uint32)), "tiny_mc_gpu_philox")
     return out
+
+
+def eval_step_math(u, math=MATH_HARDWARE):
+    """The walk's step length -ln(u) and sin/cos(2 pi u) in math mode `math`
+    on inputs u (each ((r >> 8) + 1) 2^-24): float32 arrays (t, sin, cos)."""
+    lib = load()
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    t, sn, cs = (np.zeros_like(u) for _ in range(3))
+    fp = ctypes.c_float
+    _check(lib.tiny_mc_gpu_eval_step_math(math, _ptr(u, fp), u.size, _ptr(t, fp), _ptr(sn, fp), _ptr(cs, fp)),
+           "tiny_mc_gpu_eval_step_math")
+    return t, sn, cs
+
+
+def step_math_tables(math=MATH_HARDWARE):
+    """(t, sin, cos) of the walk for every uniform it can draw, indexed by
+    r >> 8 (2^24 entries each)."""
+    u = ((np.arange(1 << 24, dtype=np.float64) + 1.0) * 2.0 ** -24).astype(np.float32)
+    return eval_step_math(u, math)
 
 
 def check_sqrt(first_bits, count):
@@ -260,7 +284,7 @@ class _BatchOptions(ctypes.Structure):
         ("chunk_work", ctypes.c_uint64),
         ("max_interactions", ctypes.c_uint64),
         ("use_graph", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("math", ctypes.c_uint32),
     ]
 
 
@@ -316,10 +340,11 @@ class BatchOptions:
     chunk_work: int = 0
     max_interactions: int = 0
     use_graph: bool = False
+    math: int = MATH_HARDWARE
 
     def _c(self):
         return _BatchOptions(self.sampler, self.block, self.waves_per_simd, self.copies, self.chunk_work,
-                             self.max_interactions, int(self.use_graph), 0)
+                             self.max_interactions, int(self.use_graph), self.math)
 
 
 def _bind_batch(lib):
@@ -436,7 +461,7 @@ def batch_autotune(sets, base: BatchOptions = None, sample_divisor=16):
     _check(lib.tmc_batch_autotune(arr, n, ctypes.byref(b), sample_divisor, ctypes.byref(out)),
            "tmc_batch_autotune")
     return BatchOptions(out.sampler, out.block, out.waves_per_simd, out.copies, out.chunk_work,
-                        out.max_interactions, bool(out.use_graph))
+                        out.max_interactions, bool(out.use_graph), out.math)
 
 
 def expected_interactions(params: Params):
@@ -466,6 +491,8 @@ class _ServeOptions(ctypes.Structure):
         ("chunk_work", ctypes.c_uint64),
         ("max_photons_per_lane", ctypes.c_uint32),
         ("idle_ms", ctypes.c_uint32),
+        ("math", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -527,12 +554,12 @@ class Server:
     """Request loop over a resident persistent kernel (tmc_serve_*)."""
 
     def __init__(self, sampler=SAMPLER_ANALYTIC, max_shells=0, max_requests=0, block=0, tally_words=0,
-                 ring_chunks=0, chunk_work=0, max_photons_per_lane=0, idle_ms=0):
+                 ring_chunks=0, chunk_work=0, max_photons_per_lane=0, idle_ms=0, math=MATH_HARDWARE):
         lib = load()
         _bind_serve(lib)
         self._lib = lib
         o = _ServeOptions(sampler, max_shells, max_requests, block, tally_words, ring_chunks, chunk_work,
-                          max_photons_per_lane, idle_ms)
+                          max_photons_per_lane, idle_ms, math, 0)
         h = ctypes.c_void_p()
         _check(lib.tmc_serve_open(ctypes.byref(o), ctypes.byref(h)), "tmc_serve_open")
         self._h = h

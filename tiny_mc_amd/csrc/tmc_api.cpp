@@ -208,7 +208,7 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
 {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t S = p->shells;
-    if (o->sampler > TMC_SAMPLER_REJECTION || o->precision > TMC_PRECISION_FP64)
+    if (o->sampler > TMC_SAMPLER_REJECTION || o->precision > TMC_PRECISION_FP64 || o->math > TMC_MATH_OWN)
         return TMC_E_INVALID;
     if (p->photons > UINT64_MAX - o->first_photon)
         return TMC_E_RANGE;
@@ -222,7 +222,7 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     uint32_t grid = o->grid;
     if (grid == 0) {
         int per_cu = 0;
-        if (tmc::walk_occupancy(o->sampler, o->precision, block, S, &per_cu) != hipSuccess || per_cu <= 0)
+        if (tmc::walk_occupancy(o->sampler, o->precision, o->math, block, S, &per_cu) != hipSuccess || per_cu <= 0)
             return TMC_E_HIP;
         grid = (uint32_t)per_cu * (uint32_t)c->compute_units;
     }
@@ -277,7 +277,7 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     if (p->photons > 0) {
         if (hipEventRecord(c->ev_start, c->stream) != hipSuccess)
             return TMC_E_HIP;
-        if (tmc::launch_walk(o->sampler, o->precision, a, grid, block, c->stream) != hipSuccess)
+        if (tmc::launch_walk(o->sampler, o->precision, o->math, a, grid, block, c->stream) != hipSuccess)
             return TMC_E_HIP;
         if (hipEventRecord(c->ev_stop, c->stream) != hipSuccess)
             return TMC_E_HIP;
@@ -301,6 +301,7 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
         st->chunk = chunk;
         st->sampler = o->sampler;
         st->precision = o->precision;
+        st->math = o->math;
     }
     return TMC_OK;
 }
@@ -461,6 +462,33 @@ TMC_EXPORT int tiny_mc_gpu_philox(const uint32_t* ctr, const uint32_t* key, uint
         || tmc::launch_philox((const uint32_t*)b.p[0], (const uint32_t*)b.p[1], n, (uint32_t*)b.p[2], c->stream)
             != hipSuccess
         || hipMemcpyAsync(out, b.p[2], 16 * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipStreamSynchronize(c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    return TMC_OK;
+}
+
+TMC_EXPORT int tiny_mc_gpu_eval_step_math(uint32_t math, const float* in, uint32_t n, float* step_out,
+                                          float* sin_out, float* cos_out)
+{
+    if (((!in || !step_out || !sin_out || !cos_out) && n) || math > TMC_MATH_OWN)
+        return TMC_E_INVALID;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Context* c = nullptr;
+    int rc = tmc::acquire_context(&c);
+    if (rc != TMC_OK || n == 0)
+        return rc;
+    DevBufs b;
+    const size_t bytes = 4 * (size_t)n;
+    for (int i = 0; i < 4; ++i)
+        if (hipMalloc(&b.p[i], bytes) != hipSuccess)
+            return TMC_E_NOMEM;
+    if (hipMemcpyAsync(b.p[0], in, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess
+        || tmc::launch_eval_step_math(math, (const float*)b.p[0], n, (float*)b.p[1], (float*)b.p[2],
+                                      (float*)b.p[3], c->stream)
+            != hipSuccess
+        || hipMemcpyAsync(step_out, b.p[1], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipMemcpyAsync(sin_out, b.p[2], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipMemcpyAsync(cos_out, b.p[3], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
         || hipStreamSynchronize(c->stream) != hipSuccess)
         return TMC_E_HIP;
     return TMC_OK;

@@ -299,7 +299,7 @@ __device__ __forceinline__ void flush_and_publish(unsigned long long* slot, cons
     absorbed = 0;
 }
 
-template <uint32_t SAMPLER, bool SERVE>
+template <uint32_t SAMPLER, bool SERVE, uint32_t MATH>
 __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
 {
     extern __shared__ unsigned long long lds_slots[];
@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
 
         bool truncated = false;
         if (alive) {
-            alive = step<SAMPLER>(ph, st.phys, lt);
+            alive = step<SAMPLER, MATH>(ph, st.phys, lt);
             if (alive && ph.k >= st.kcap) {
                 alive = false;
                 truncated = true;
@@ -447,23 +447,34 @@ size_t batch_lds_bytes(uint32_t block, uint32_t shells_max, uint32_t copies)
     return (size_t)(block / 64) * slot_words(shells_max, copies) * sizeof(unsigned long long);
 }
 
-hipError_t launch_batch(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream)
+namespace {
+
+using MultiKernel = void (*)(BatchArgs);
+
+template <bool SERVE>
+MultiKernel multi_kernel(uint32_t sampler, uint32_t math)
+{
+    const bool rej = sampler == 1u;
+    if (math == TMC_MATH_OWN)
+        return rej ? walk_multi<1u, SERVE, TMC_MATH_OWN> : walk_multi<0u, SERVE, TMC_MATH_OWN>;
+    return rej ? walk_multi<1u, SERVE, TMC_MATH_HARDWARE> : walk_multi<0u, SERVE, TMC_MATH_HARDWARE>;
+}
+
+} // namespace
+
+hipError_t launch_batch(uint32_t sampler, uint32_t math, const BatchArgs& a, uint32_t grid, uint32_t block,
+                        hipStream_t stream)
 {
     const size_t lds = batch_lds_bytes(block, a.shells_max, a.copies);
-    if (sampler == 1u)
-        hipLaunchKernelGGL((walk_multi<1u, false>), dim3(grid), dim3(block), lds, stream, a);
-    else
-        hipLaunchKernelGGL((walk_multi<0u, false>), dim3(grid), dim3(block), lds, stream, a);
+    hipLaunchKernelGGL(multi_kernel<false>(sampler, math), dim3(grid), dim3(block), lds, stream, a);
     return hipGetLastError();
 }
 
-hipError_t launch_serve(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream)
+hipError_t launch_serve(uint32_t sampler, uint32_t math, const BatchArgs& a, uint32_t grid, uint32_t block,
+                        hipStream_t stream)
 {
     const size_t lds = batch_lds_bytes(block, a.shells_max, a.copies);
-    if (sampler == 1u)
-        hipLaunchKernelGGL((walk_multi<1u, true>), dim3(grid), dim3(block), lds, stream, a);
-    else
-        hipLaunchKernelGGL((walk_multi<0u, true>), dim3(grid), dim3(block), lds, stream, a);
+    hipLaunchKernelGGL(multi_kernel<true>(sampler, math), dim3(grid), dim3(block), lds, stream, a);
     return hipGetLastError();
 }
 
@@ -474,20 +485,20 @@ hipError_t launch_batch_clear(unsigned long long* tally, uint64_t tally_words, u
     return hipGetLastError();
 }
 
-hipError_t batch_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max, uint32_t copies, int* wg_per_cu)
+hipError_t batch_occupancy(uint32_t sampler, uint32_t math, uint32_t block, uint32_t shells_max, uint32_t copies,
+                           int* wg_per_cu)
 {
     const size_t lds = batch_lds_bytes(block, shells_max, copies);
-    if (sampler == 1u)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_multi<1u, false>, (int)block, lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_multi<0u, false>, (int)block, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, multi_kernel<false>(sampler, math), (int)block,
+                                                        lds);
 }
 
-hipError_t serve_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max, uint32_t copies, int* wg_per_cu)
+hipError_t serve_occupancy(uint32_t sampler, uint32_t math, uint32_t block, uint32_t shells_max, uint32_t copies,
+                           int* wg_per_cu)
 {
     const size_t lds = batch_lds_bytes(block, shells_max, copies);
-    if (sampler == 1u)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_multi<1u, true>, (int)block, lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_multi<0u, true>, (int)block, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, multi_kernel<true>(sampler, math), (int)block,
+                                                        lds);
 }
 
 } // namespace tmc

@@ -35,7 +35,7 @@ struct tmc_batch_plan {
     uint64_t photons = 0;
     double expected = 0.0;
     uint64_t chunk_work = 0;
-    uint32_t sampler = 0, grid = 0, block = 0, copies = 1, wps = 0, shells_max = 1;
+    uint32_t sampler = 0, math = 0, grid = 0, block = 0, copies = 1, wps = 0, shells_max = 1;
     bool use_graph = false;
     tmc::BatchSet* d_sets = nullptr;
     tmc::BatchChunk* d_chunks = nullptr;
@@ -83,7 +83,7 @@ hipError_t enqueue_run(tmc_batch_plan* p, hipStream_t stream)
     if (e != hipSuccess)
         return e;
     if (p->nchunks) {
-        e = tmc::launch_batch(p->sampler, p->args, p->grid, p->block, stream);
+        e = tmc::launch_batch(p->sampler, p->math, p->args, p->grid, p->block, stream);
         if (e != hipSuccess)
             return e;
     }
@@ -97,7 +97,7 @@ hipError_t enqueue_run(tmc_batch_plan* p, hipStream_t stream)
 int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, const tmc_batch_options* o,
                        tmc_batch_plan** out)
 {
-    if (o->sampler > TMC_SAMPLER_REJECTION)
+    if (o->sampler > TMC_SAMPLER_REJECTION || o->math > TMC_MATH_OWN)
         return TMC_E_INVALID;
     if (o->block && (o->block < 64u || o->block > 1024u || o->block % 64u))
         return TMC_E_INVALID;
@@ -112,6 +112,7 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
         return TMC_E_NOMEM;
     p->nsets = nsets;
     p->sampler = o->sampler;
+    p->math = o->math;
     p->copies = copies;
     p->use_graph = o->use_graph != 0;
 
@@ -158,7 +159,7 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
     for (uint32_t b : cand) {
         int n = 0;
         if (tmc::batch_lds_bytes(b, shells_max, copies) <= 160u * 1024u
-            && tmc::batch_occupancy(p->sampler, b, shells_max, copies, &n) != hipSuccess) {
+            && tmc::batch_occupancy(p->sampler, p->math, b, shells_max, copies, &n) != hipSuccess) {
             plan_free(p);
             return TMC_E_HIP;
         }

@@ -84,12 +84,16 @@ struct BatchArgs {
 
 size_t batch_lds_bytes(uint32_t block, uint32_t shells_max, uint32_t copies);
 
-hipError_t launch_batch(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
+hipError_t launch_batch(uint32_t sampler, uint32_t math, const BatchArgs& a, uint32_t grid, uint32_t block,
+                        hipStream_t stream);
 hipError_t launch_batch_clear(unsigned long long* tally, uint64_t tally_words, unsigned long long* done,
                               uint32_t nsets, unsigned long long* counters, hipStream_t stream);
-hipError_t launch_serve(uint32_t sampler, const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
-hipError_t serve_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max, uint32_t copies, int* wg_per_cu);
-hipError_t batch_occupancy(uint32_t sampler, uint32_t block, uint32_t shells_max, uint32_t copies, int* wg_per_cu);
+hipError_t launch_serve(uint32_t sampler, uint32_t math, const BatchArgs& a, uint32_t grid, uint32_t block,
+                        hipStream_t stream);
+hipError_t serve_occupancy(uint32_t sampler, uint32_t math, uint32_t block, uint32_t shells_max, uint32_t copies,
+                           int* wg_per_cu);
+hipError_t batch_occupancy(uint32_t sampler, uint32_t math, uint32_t block, uint32_t shells_max, uint32_t copies,
+                           int* wg_per_cu);
 
 } // namespace tmc
 

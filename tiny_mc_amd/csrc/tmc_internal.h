@@ -35,14 +35,17 @@ struct WalkArgs {
 
 /* Launches the persistent walk kernel of `precision` (TMC_PRECISION_*) on
  * `stream`.  LDS: 16 bytes per shell plus the overflow shell's lane copies. */
-hipError_t launch_walk(uint32_t sampler, uint32_t precision, const WalkArgs& a, uint32_t grid, uint32_t block,
-                       hipStream_t stream);
+hipError_t launch_walk(uint32_t sampler, uint32_t precision, uint32_t math, const WalkArgs& a, uint32_t grid,
+                       uint32_t block, hipStream_t stream);
 
 /* Resident workgroups per CU for the walk kernel at this block size. */
-hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t block, uint32_t shells, int* wg_per_cu);
+hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t math, uint32_t block, uint32_t shells,
+                          int* wg_per_cu);
 
 hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out, hipStream_t stream);
 hipError_t launch_check_sqrt(uint32_t first, uint64_t count, unsigned long long* out, hipStream_t stream);
+hipError_t launch_eval_step_math(uint32_t math, const float* in, uint32_t n, float* t, float* sn, float* cs,
+                                 hipStream_t stream);
 hipError_t launch_eval_math(const float* in, uint32_t n, float* lg, float* sn, float* cs, hipStream_t stream);
 
 } // namespace tmc

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/tiny_mc_gpu.h"
+
 namespace tmc {
 
 #define TMC_LN2_HI 0x1.62e400p-1f /* 17 significant bits: e * LN2_HI is exact */
@@ -98,6 +100,33 @@ __device__ __forceinline__ void sincos2pi32(float a, float* s_out, float* c_out)
     /* quadrant 0: ( s,  c)  1: ( c, -s)  2: (-s, -c)  3: (-c,  s) */
     *s_out = (q & 2) ? -s0 : s0;
     *c_out = ((q + 1) & 2) ? -c0 : c0;
+}
+
+/* The two transcendental quantities of a step, by math mode:
+ *   TMC_MATH_HARDWARE: the hardware base-2 log (v_log_f32) scaled by -ln 2,
+ *     and the hardware sine and cosine of a turn fraction (v_sin_f32 /
+ *     v_cos_f32 take their argument in revolutions);
+ *   TMC_MATH_OWN: the polynomials above, which a host program reproduces
+ *     without the device.
+ * r is the 32-bit Philox word of the uniform u = ((r >> 8) + 1) 2^-24. */
+template <uint32_t MATH>
+__device__ __forceinline__ float step_length(uint32_t r)
+{
+    if (MATH == TMC_MATH_HARDWARE)
+        return __builtin_amdgcn_logf((float)((r >> 8) + 1u) * 0x1p-24f) * -0x1.62e430p-1f;
+    return -log32_u24((float)((r >> 8) + 1u)); /* = -log32(u) */
+}
+
+template <uint32_t MATH>
+__device__ __forceinline__ void sincos_turn(uint32_t r, float* s_out, float* c_out)
+{
+    const float a = (float)((r >> 8) + 1u) * 0x1p-24f;
+    if (MATH == TMC_MATH_HARDWARE) {
+        *s_out = __builtin_amdgcn_sinf(a);
+        *c_out = __builtin_amdgcn_cosf(a);
+    } else {
+        sincos2pi32(a, s_out, c_out);
+    }
 }
 
 /* Correctly rounded square root (IEEE), the compiler's default lowering of

@@ -312,7 +312,7 @@ void serve_step(tmc_server* s)
     if (!s->kernel_running && s->published > s->base) {
         s->args.ticket_base = s->base;
         s->args.launch_id = ++s->launch_id;
-        if (tmc::launch_serve(s->opt.sampler, s->args, s->grid, s->block, s->kstream) != hipSuccess) {
+        if (tmc::launch_serve(s->opt.sampler, s->opt.math, s->args, s->grid, s->block, s->kstream) != hipSuccess) {
             s->error = TMC_E_HIP;
             return;
         }
@@ -420,7 +420,7 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
         o = *opt;
     else
         tmc_serve_options_default(&o);
-    if (o.sampler > TMC_SAMPLER_REJECTION)
+    if (o.sampler > TMC_SAMPLER_REJECTION || o.math > TMC_MATH_OWN)
         return TMC_E_INVALID;
     if (!o.max_shells)
         o.max_shells = 401;
@@ -455,7 +455,7 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
     std::memset(&s->st, 0, sizeof s->st);
     int wg = 0;
     if (tmc::batch_lds_bytes(o.block, o.max_shells, 1) > 160u * 1024u
-        || tmc::serve_occupancy(o.sampler, o.block, o.max_shells, 1, &wg) != hipSuccess || wg <= 0) {
+        || tmc::serve_occupancy(o.sampler, o.math, o.block, o.max_shells, 1, &wg) != hipSuccess || wg <= 0) {
         delete s;
         return TMC_E_INVALID;
     }

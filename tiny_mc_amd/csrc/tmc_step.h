@@ -54,14 +54,16 @@ struct LaneTally {
 };
 
 /* Walks one absorption event of p, adding its deposit to the fixed-point
- * shell sums in LDS.  Returns false when the roulette kills the photon. */
-template <uint32_t SAMPLER>
+ * shell sums in LDS.  Returns false when the roulette kills the photon.
+ * MATH (TMC_MATH_*) selects how -ln u and sin/cos(2 pi u') are evaluated;
+ * everything else is the same arithmetic in both modes. */
+template <uint32_t SAMPLER, uint32_t MATH>
 __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt)
 {
     const tmc_u32x4 r = tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1); /* block (k, 0) */
 
     /* move (photon.c:21-24) */
-    const float t = -log32_u24(tmc_u24(r.x0)); /* = -log32(tmc_u01(r.x0)) */
+    const float t = step_length<MATH>(r.x0);
     p.x = __builtin_fmaf(t, p.u, p.x);
     p.y = __builtin_fmaf(t, p.v, p.y);
     p.z = __builtin_fmaf(t, p.w, p.z);
@@ -104,7 +106,7 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
         const float mu = __builtin_fmaf(tmc_u24(r.x1), 0x1p-23f, -1.0f); /* = 2 u - 1, both exact */
         const float s = sqrt_cr_walk(__builtin_fmaf(-mu, mu, 1.0f)); /* 0 or >= 2^-23 */
         float sn, cs;
-        sincos2pi32(tmc_u01(r.x2), &sn, &cs);
+        sincos_turn<MATH>(r.x2, &sn, &cs);
         p.u = mu;
         p.v = s * cs;
         p.w = s * sn;

@@ -69,26 +69,28 @@ __device__ __forceinline__ void flush_tallies(unsigned long long* lw, unsigned l
 
 /* One absorption event, by photon state type: the reference's FP32 step or
  * one of the precision variants (tmc_step_prec.h). */
-template <uint32_t SAMPLER, bool COMP>
+/* VARIANT: the math mode for the binary32 walk, compensation for the
+ * weight-from-counts walk, unused for binary64. */
+template <uint32_t SAMPLER, uint32_t VARIANT>
 __device__ __forceinline__ bool step_of(Photon& p, const Physics& ph, const LaneTally& lt)
 {
-    return step<SAMPLER>(p, ph, lt);
+    return step<SAMPLER, VARIANT>(p, ph, lt);
 }
 
-template <uint32_t SAMPLER, bool COMP>
+template <uint32_t SAMPLER, uint32_t VARIANT>
 __device__ __forceinline__ bool step_of(PhotonW& p, const PhysicsW& ph, const LaneTally& lt)
 {
-    return step<SAMPLER, COMP>(p, ph, lt);
+    return step<SAMPLER, VARIANT != 0u>(p, ph, lt);
 }
 
-template <uint32_t SAMPLER, bool COMP>
+template <uint32_t SAMPLER, uint32_t VARIANT>
 __device__ __forceinline__ bool step_of(Photon64& p, const Physics64& ph, const LaneTally& lt)
 {
     return step<SAMPLER>(p, ph, lt);
 }
 
 /* The persistent walk over photons [first_photon, first_photon + photons). */
-template <class PhotonT, class PhysicsT, uint32_t SAMPLER, bool COMP>
+template <class PhotonT, class PhysicsT, uint32_t SAMPLER, uint32_t VARIANT>
 __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phys)
 {
     extern __shared__ unsigned long long lds_tally[];
@@ -163,7 +165,7 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
 
         bool truncated = false;
         if (alive) {
-            alive = step_of<SAMPLER, COMP>(ph, phys, lt);
+            alive = step_of<SAMPLER, VARIANT>(ph, phys, lt);
             if (alive && ph.k >= a.kcap) {
                 alive = false;
                 truncated = true;
@@ -185,11 +187,11 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
 }
 
 /* The reference's FP32 arithmetic (the production walk). */
-template <uint32_t SAMPLER>
+template <uint32_t SAMPLER, uint32_t MATH>
 __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
 {
     const Physics phys = { a.albedo, a.spm, a.shell_max, a.shells - 1, (uint32_t)a.seed, (uint32_t)(a.seed >> 32) };
-    walk_body<Photon, Physics, SAMPLER, false>(a, phys);
+    walk_body<Photon, Physics, SAMPLER, MATH>(a, phys);
 }
 
 /* FP32 transport, weight from exact counts (COMP: compensated positions). */
@@ -198,7 +200,7 @@ __global__ void __launch_bounds__(1024) walk_f32w(WalkArgs a)
 {
     const PhysicsW phys = { a.l2a_hi, a.l2a_lo, a.l2s_hi, a.l2s_lo, a.spm_w, a.shell_max, a.shells - 1,
                             (uint32_t)a.seed, (uint32_t)(a.seed >> 32) };
-    walk_body<PhotonW, PhysicsW, SAMPLER, COMP>(a, phys);
+    walk_body<PhotonW, PhysicsW, SAMPLER, COMP ? 1u : 0u>(a, phys);
 }
 
 /* All-FP64 transport. */
@@ -207,38 +209,42 @@ __global__ void __launch_bounds__(1024) walk_f64(WalkArgs a)
 {
     const Physics64 phys = { a.albedo_d, a.spm_d, (double)(a.shells - 1), a.shells - 1, (uint32_t)a.seed,
                              (uint32_t)(a.seed >> 32) };
-    walk_body<Photon64, Physics64, SAMPLER, false>(a, phys);
+    walk_body<Photon64, Physics64, SAMPLER, 0u>(a, phys);
 }
 
 namespace {
 
 using WalkKernel = void (*)(WalkArgs);
 
-WalkKernel walk_kernel(uint32_t sampler, uint32_t precision)
+WalkKernel walk_kernel(uint32_t sampler, uint32_t precision, uint32_t math)
 {
     const bool rej = sampler == 1u;
     switch (precision) {
     case 1u: return rej ? walk_f32w<1u, false> : walk_f32w<0u, false>;
     case 2u: return rej ? walk_f32w<1u, true> : walk_f32w<0u, true>;
     case 3u: return rej ? walk_f64<1u> : walk_f64<0u>;
-    default: return rej ? walk_f32<1u> : walk_f32<0u>;
+    default:
+        if (math == TMC_MATH_OWN)
+            return rej ? walk_f32<1u, TMC_MATH_OWN> : walk_f32<0u, TMC_MATH_OWN>;
+        return rej ? walk_f32<1u, TMC_MATH_HARDWARE> : walk_f32<0u, TMC_MATH_HARDWARE>;
     }
 }
 
 } // namespace
 
-hipError_t launch_walk(uint32_t sampler, uint32_t precision, const WalkArgs& a, uint32_t grid, uint32_t block,
-                       hipStream_t stream)
+hipError_t launch_walk(uint32_t sampler, uint32_t precision, uint32_t math, const WalkArgs& a, uint32_t grid,
+                       uint32_t block, hipStream_t stream)
 {
     const size_t lds = walk_lds_words(a.shells) * sizeof(unsigned long long);
-    hipLaunchKernelGGL(walk_kernel(sampler, precision), dim3(grid), dim3(block), lds, stream, a);
+    hipLaunchKernelGGL(walk_kernel(sampler, precision, math), dim3(grid), dim3(block), lds, stream, a);
     return hipGetLastError();
 }
 
-hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t block, uint32_t shells, int* wg_per_cu)
+hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t math, uint32_t block, uint32_t shells,
+                          int* wg_per_cu)
 {
     const size_t lds = walk_lds_words(shells) * sizeof(unsigned long long);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_kernel(sampler, precision), (int)block,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_kernel(sampler, precision, math), (int)block,
                                                         lds);
 }
 
@@ -294,6 +300,37 @@ hipError_t launch_check_sqrt(uint32_t first, uint64_t count, unsigned long long*
     if (count == 0)
         return hipSuccess;
     hipLaunchKernelGGL(check_sqrt_kernel, dim3(8192), dim3(256), 0, stream, first, count, out);
+    return hipGetLastError();
+}
+
+/* The step length and sin/cos(2 pi u) of a binary32 walk in math mode MATH
+ * on inputs u = in[i] (each of the form ((r >> 8) + 1) 2^-24). */
+template <uint32_t MATH>
+__global__ void eval_step_math_kernel(const float* in, uint32_t n, float* t, float* sn, float* cs)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    /* recover r >> 8 from u: u * 2^24 - 1 is an exact integer in [0, 2^24) */
+    const uint32_t r = ((uint32_t)(in[i] * 0x1p24f) - 1u) << 8;
+    t[i] = step_length<MATH>(r);
+    float s, c;
+    sincos_turn<MATH>(r, &s, &c);
+    sn[i] = s;
+    cs[i] = c;
+}
+
+hipError_t launch_eval_step_math(uint32_t math, const float* in, uint32_t n, float* t, float* sn, float* cs,
+                                 hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    if (math == TMC_MATH_OWN)
+        hipLaunchKernelGGL(eval_step_math_kernel<TMC_MATH_OWN>, dim3((n + 255) / 256), dim3(256), 0, stream, in, n,
+                           t, sn, cs);
+    else
+        hipLaunchKernelGGL(eval_step_math_kernel<TMC_MATH_HARDWARE>, dim3((n + 255) / 256), dim3(256), 0, stream,
+                           in, n, t, sn, cs);
     return hipGetLastError();
 }
 

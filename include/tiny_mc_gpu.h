@@ -152,16 +152,29 @@ const char* tiny_mc_gpu_strerror(int status);
 int tiny_mc_gpu_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out);
 int tiny_mc_gpu_eval_math(const float* in, uint32_t n, float* log_out, float* sin_out, float* cos_out);
 
-/* The walk's own step length -ln(u) and sin/cos(2 pi u) in math mode `math`
+/* The walk's own step length -ln(u), sin/cos(2 pi u) and the analytic
+ * sampler's sin(theta) = sqrt(1 - (2u - 1)^2) in math mode `math`
  * (TMC_MATH_*) on n inputs: the exact values a binary32 walk uses. */
 int tiny_mc_gpu_eval_step_math(uint32_t math, const float* in, uint32_t n, float* step_out, float* sin_out,
-                               float* cos_out);
+                               float* cos_out, float* polar_out);
 
 /* Diagnostic: compares the walk's short correctly rounded square root with
  * the device's IEEE square root on every float whose bit pattern lies in
  * [first_bits, first_bits + count); returns the number of mismatches and
  * the lowest mismatching pattern (0 if none). */
 int tiny_mc_gpu_check_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismatches, uint32_t* first_mismatch);
+
+/* Diagnostic: the same comparison for each short form the walk uses in
+ * place of an IEEE operation:
+ *   TMC_CHECK_SQRT             the short square root (= tiny_mc_gpu_check_sqrt);
+ *   TMC_CHECK_DIV_TENTH_SHORT  fma(w, 10, w * C_lo) against w / 0.1f (the
+ *                              roulette's weight update), exact for w >= 2^-100;
+ *   TMC_CHECK_DIV_TENTH        the guarded form the walk calls, exact for all w. */
+#define TMC_CHECK_SQRT 0u
+#define TMC_CHECK_DIV_TENTH_SHORT 1u
+#define TMC_CHECK_DIV_TENTH 2u
+int tiny_mc_gpu_check_short_op(uint32_t op, uint32_t first_bits, uint64_t count, uint64_t* mismatches,
+                               uint32_t* first_mismatch);
 
 /* ------------------------------------------------------------------------
  * Batched sweep: many parameter sets in one persistent launch.

@@ -439,8 +439,9 @@ typedef struct m32_params {
     uint32_t shells;
     int sampler;
     uint64_t max_k;
-    /* NULL: own log / sin / cos; else per uniform index r >> 8 */
-    const float *t_tab, *s_tab, *c_tab;
+    /* NULL: own log / sin / cos and the correctly rounded sin(theta); else
+     * per uniform index r >> 8 */
+    const float *t_tab, *s_tab, *c_tab, *p_tab;
 } m32_params;
 
 /* Fixed-point images of one deposit: w * 2^33 and (w*w) * 2^43, truncated.
@@ -500,7 +501,7 @@ static uint64_t mirror_photon(uint64_t seed, uint64_t photon, const m32_params* 
             w = xi2 * s;
         } else {
             const float mu = fmaf(2.0f, orc_u01(b0[1]), -1.0f);
-            const float s = sqrtf(fmaf(-mu, mu, 1.0f));
+            const float s = mp->p_tab ? mp->p_tab[b0[1] >> 8] : sqrtf(fmaf(-mu, mu, 1.0f));
             float sn, cs;
             if (mp->s_tab) {
                 sn = mp->s_tab[b0[2] >> 8];
@@ -554,12 +555,12 @@ ORC_EXPORT void orc_derived32(const orc_params* p, float* albedo, float* spm)
  * (lo, hi) of the 128-bit sum of (w*w)*2^43 for each shell. */
 ORC_EXPORT int orc_mirror32_tab(const orc_params* p, uint64_t seed, uint64_t first, uint64_t count, int sampler,
                                 uint64_t max_k, int threads, const float* t_tab, const float* s_tab,
-                                const float* c_tab, uint64_t* tally, uint64_t* interactions)
+                                const float* c_tab, const float* p_tab, uint64_t* tally, uint64_t* interactions)
 {
     if (!orc_params_ok(p) || !tally || (sampler != ORC_SAMPLER_ANALYTIC && sampler != ORC_SAMPLER_REJECTION)
-        || (t_tab && (!s_tab || !c_tab)))
+        || (t_tab && (!s_tab || !c_tab || !p_tab)))
         return -1;
-    m32_params mp = { restate_albedo(p), restate_spm(p), p->shells, sampler, max_k, t_tab, s_tab, c_tab };
+    m32_params mp = { restate_albedo(p), restate_spm(p), p->shells, sampler, max_k, t_tab, s_tab, c_tab, p_tab };
     threads = clamp_threads(threads, count);
     const size_t S = p->shells;
     u128* acc = calloc((size_t)threads * 2 * S, sizeof(u128));
@@ -597,11 +598,13 @@ ORC_EXPORT int orc_mirror32_tab(const orc_params* p, uint64_t seed, uint64_t fir
     return 0;
 }
 
-/* orc_mirror32_tab with the own log / sin / cos polynomials. */
+/* orc_mirror32_tab with the own log / sin / cos polynomials and correctly
+ * rounded square roots. */
 ORC_EXPORT int orc_mirror32(const orc_params* p, uint64_t seed, uint64_t first, uint64_t count, int sampler,
                             uint64_t max_k, int threads, uint64_t* tally, uint64_t* interactions)
 {
-    return orc_mirror32_tab(p, seed, first, count, sampler, max_k, threads, NULL, NULL, NULL, tally, interactions);
+    return orc_mirror32_tab(p, seed, first, count, sampler, max_k, threads, NULL, NULL, NULL, NULL, tally,
+                            interactions);
 }
 
 /* ------------------------------------------------------------------------ */

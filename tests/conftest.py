@@ -33,6 +33,7 @@ def tmc():
 
 @pytest.fixture(scope="session")
 def hw_tables(tmc):
-    """The device's step length and sin/cos for every uniform the walk can
-    draw in the hardware-math mode (indexed by r >> 8), for the mirror."""
+    """The device's step length, sin/cos and sin(theta) for every uniform the
+    walk can draw in the hardware-math mode (indexed by r >> 8), for the
+    mirror."""
     return tmc.step_math_tables(tmc.MATH_HARDWARE)

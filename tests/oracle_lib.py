@@ -51,7 +51,7 @@ class Oracle:
         L.orc_mirror32.restype = ctypes.c_int
         fp = P(ctypes.c_float)
         L.orc_mirror32_tab.argtypes = [P(OrcParams), ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
-                                       ctypes.c_uint64, ctypes.c_int, fp, fp, fp, P(ctypes.c_uint64),
+                                       ctypes.c_uint64, ctypes.c_int, fp, fp, fp, fp, P(ctypes.c_uint64),
                                        P(ctypes.c_uint64)]
         L.orc_mirror32_tab.restype = ctypes.c_int
         L.orc_philox.argtypes = [P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32)]
@@ -109,9 +109,9 @@ class Oracle:
         return h, h2, n.value
 
     def mirror32(self, p, seed, count, first=0, sampler=SAMPLER_ANALYTIC, max_k=0, nthreads=None, tables=None):
-        """FP32 mirror; tables = (t, sin, cos) float32 arrays of 2^24 entries
-        indexed by r >> 8 for the kernel's hardware-math mode, None for the
-        own polynomials."""
+        """FP32 mirror; tables = (t, sin, cos, sin_theta) float32 arrays of
+        2^24 entries indexed by r >> 8 for the kernel's hardware-math mode,
+        None for the own polynomials."""
         t = np.zeros(4 * p.shells, np.uint64)
         n = ctypes.c_uint64()
         u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -119,12 +119,12 @@ class Oracle:
             rc = self.L.orc_mirror32(ctypes.byref(p), seed, first, count, sampler, max_k, nthreads or threads(),
                                      t.ctypes.data_as(u64p), ctypes.byref(n))
         else:
-            tt, ts, tc = (np.ascontiguousarray(a, dtype=np.float32) for a in tables)
-            assert tt.size == ts.size == tc.size == 1 << 24
+            tt, ts, tc, tp = (np.ascontiguousarray(a, dtype=np.float32) for a in tables)
+            assert tt.size == ts.size == tc.size == tp.size == 1 << 24
             fp = ctypes.POINTER(ctypes.c_float)
             rc = self.L.orc_mirror32_tab(ctypes.byref(p), seed, first, count, sampler, max_k, nthreads or threads(),
                                          tt.ctypes.data_as(fp), ts.ctypes.data_as(fp), tc.ctypes.data_as(fp),
-                                         t.ctypes.data_as(u64p), ctypes.byref(n))
+                                         tp.ctypes.data_as(fp), t.ctypes.data_as(u64p), ctypes.byref(n))
         assert rc == 0
         return t, n.value
 

@@ -148,24 +148,50 @@ def test_short_sqrt_exhaustive(tmc):
     assert tmc.check_sqrt(lo, hi - lo + 1) == (0, 0)
 
 
+def test_roulette_division_exhaustive(tmc):
+    """The roulette's weight update w / 0.1f as fma(w, 10, w * C_lo) equals
+    the IEEE quotient on every float of magnitude from 2^-100 to the largest
+    finite one, and the guarded form the walk calls equals it on every
+    finite float (zeros and subnormals included; lanes without a photon
+    carry negative weights through the step)."""
+    lo = int(np.float32(2.0 ** -100).view(np.uint32))
+    inf = int(np.float32(np.inf).view(np.uint32))
+    sign = 1 << 31
+    for base in (0, sign):
+        assert tmc.check_short_op(tmc.CHECK_DIV_TENTH_SHORT, base + lo, inf - lo) == (0, 0)
+        assert tmc.check_short_op(tmc.CHECK_DIV_TENTH, base, inf) == (0, 0)
+    # the short form alone is not exact everywhere: the guard is needed
+    n, first = tmc.check_short_op(tmc.CHECK_DIV_TENTH_SHORT, 0, lo)
+    assert n > 0 and first > 0
+
+
 def test_hardware_math_accuracy_exhaustive(tmc, orc, hw_tables):
-    """Hardware-math step length -ln(u) within 2 ulp and sin/cos(2 pi u)
-    within 2e-7 of float64 on every uniform the walk can draw; the own-math
-    evaluation equals the host polynomials bit for bit."""
-    t, sn, cs = hw_tables
+    """Hardware-math step length -ln(u) within 2 ulp, sin/cos(2 pi u) within
+    2e-7 and the analytic sampler's sin(theta) = sqrt(1 - (2u - 1)^2)
+    within 2 ulp of float64 on every uniform the walk can draw; the own-math
+    evaluation equals the host polynomials and the correctly rounded
+    sin(theta) bit for bit."""
+    t, sn, cs, sp = hw_tables
     m = np.arange(1, (1 << 24) + 1, dtype=np.float64)
     u = m * 2.0 ** -24
-    ref = -np.log(u)
-    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
-    ulp[ulp == 0] = np.spacing(np.float32(0))
-    assert np.max(np.abs(t - ref) / ulp) <= 2.0
+
+    def ulps(got, ref):
+        ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+        ulp[ulp == 0] = np.spacing(np.float32(0))
+        return np.abs(got - ref) / ulp
+
+    assert np.max(ulps(t, -np.log(u))) <= 2.0
     assert np.max(np.abs(sn - np.sin(2 * np.pi * u))) <= 2e-7
     assert np.max(np.abs(cs - np.cos(2 * np.pi * u))) <= 2e-7
+    mu = 2.0 * u - 1.0  # exact, as is 1 - mu^2
+    assert np.max(ulps(sp, np.sqrt(1.0 - mu * mu))) <= 2.0
     # the own polynomials as the walk evaluates them, against the host's
     idx = np.arange(0, 1 << 24, 97)
     uf = u[idx].astype(np.float32)
-    ot, osn, ocs = tmc.eval_step_math(uf, tmc.MATH_OWN)
+    ot, osn, ocs, osp = tmc.eval_step_math(uf, tmc.MATH_OWN)
     lg, hs, hc = orc.eval_math32(uf)
     np.testing.assert_array_equal(ot.view(np.uint32), (-lg).view(np.uint32))
     np.testing.assert_array_equal(osn.view(np.uint32), hs.view(np.uint32))
     np.testing.assert_array_equal(ocs.view(np.uint32), hc.view(np.uint32))
+    want_sp = np.sqrt((1.0 - mu[idx] * mu[idx]).astype(np.float32))
+    np.testing.assert_array_equal(osp.view(np.uint32), want_sp.view(np.uint32))

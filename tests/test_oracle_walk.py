@@ -148,16 +148,24 @@ def test_derived_constants_match_reference_arithmetic(orc):
 @pytest.mark.parametrize("sampler", [0, 1])
 def test_mirror_table_mode_equals_own_math(orc, sampler):
     """The mirror's table mode (used for the kernel's hardware math) looks up
-    entry r >> 8: fed the own polynomials' values it reproduces the
-    own-math mirror bit for bit."""
-    u = ((np.arange(1 << 24, dtype=np.float64) + 1.0) * 2.0 ** -24).astype(np.float32)
+    entry r >> 8: fed the own polynomials' values and the correctly rounded
+    sin(theta) it reproduces the own-math mirror bit for bit."""
+    u64 = (np.arange(1 << 24, dtype=np.float64) + 1.0) * 2.0 ** -24
+    u = u64.astype(np.float32)
     lg, sn, cs = orc.eval_math32(u)
-    tables = (-lg, sn, cs)
+    # mu = 2u - 1 and 1 - mu^2 are exact in float64; one rounding to float32
+    # is the FMA's, and numpy's float32 sqrt is correctly rounded
+    mu = 2.0 * u64 - 1.0
+    sp = np.sqrt((1.0 - mu * mu).astype(np.float32))
+    tables = (-lg, sn, cs, sp)
     p = orc.params()
     a, na = orc.mirror32(p, 77, 4096, sampler=sampler)
     b, nb = orc.mirror32(p, 77, 4096, sampler=sampler, tables=tables)
     np.testing.assert_array_equal(a, b)
     assert na == nb
-    # a table that differs from the polynomials changes the tallies
-    c, _ = orc.mirror32(p, 77, 4096, sampler=sampler, tables=(-lg * np.float32(1.001), sn, cs))
+    # a table that differs from the polynomials changes the tallies (the
+    # rejection sampler does not read sin(theta))
+    c, _ = orc.mirror32(p, 77, 4096, sampler=sampler, tables=(-lg * np.float32(1.001), sn, cs, sp))
     assert not np.array_equal(a, c)
+    d, _ = orc.mirror32(p, 77, 4096, sampler=sampler, tables=(-lg, sn, cs, sp * np.float32(1.001)))
+    assert np.array_equal(a, d) == (sampler == 1)

@@ -32,6 +32,10 @@ from ._lib import (  # noqa: F401
     RunStats,
     TmcError,
     check_sqrt,
+    check_short_op,
+    CHECK_SQRT,
+    CHECK_DIV_TENTH_SHORT,
+    CHECK_DIV_TENTH,
     device_info,
     eval_math,
     eval_step_math,

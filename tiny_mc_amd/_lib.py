@@ -143,10 +143,12 @@ def load():
     lib.tiny_mc_gpu_philox.argtypes = [u32p, u32p, ctypes.c_uint32, u32p]
     lib.tiny_mc_gpu_eval_math.argtypes = [fp, ctypes.c_uint32, fp, fp, fp]
     lib.tiny_mc_gpu_check_sqrt.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u64p, P(ctypes.c_uint32)]
-    lib.tiny_mc_gpu_eval_step_math.argtypes = [ctypes.c_uint32, fp, ctypes.c_uint32, fp, fp, fp]
+    lib.tiny_mc_gpu_check_short_op.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, u64p,
+                                               P(ctypes.c_uint32)]
+    lib.tiny_mc_gpu_eval_step_math.argtypes = [ctypes.c_uint32, fp, ctypes.c_uint32, fp, fp, fp, fp]
     for name in ("tiny_mc_gpu_run", "tiny_mc_gpu_run_ex", "tiny_mc_gpu_tally_to_heat", "tiny_mc_gpu_init",
                  "tiny_mc_gpu_device_info", "tiny_mc_gpu_philox", "tiny_mc_gpu_eval_math", "tiny_mc_gpu_check_sqrt",
-                 "tiny_mc_gpu_eval_step_math"):
+                 "tiny_mc_gpu_eval_step_math", "tiny_mc_gpu_check_short_op"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -213,20 +215,22 @@ def philox(ctr, key):
 
 
 def eval_step_math(u, math=MATH_HARDWARE):
-    """The walk's step length -ln(u) and sin/cos(2 pi u) in math mode `math`
-    on inputs u (each ((r >> 8) + 1) 2^-24): float32 arrays (t, sin, cos)."""
+    """The walk's step length -ln(u), sin/cos(2 pi u) and the analytic
+    sampler's sin(theta) = sqrt(1 - (2u - 1)^2) in math mode `math` on
+    inputs u (each ((r >> 8) + 1) 2^-24): float32 arrays (t, sin, cos,
+    sin_theta)."""
     lib = load()
     u = np.ascontiguousarray(u, dtype=np.float32)
-    t, sn, cs = (np.zeros_like(u) for _ in range(3))
+    t, sn, cs, sp = (np.zeros_like(u) for _ in range(4))
     fp = ctypes.c_float
-    _check(lib.tiny_mc_gpu_eval_step_math(math, _ptr(u, fp), u.size, _ptr(t, fp), _ptr(sn, fp), _ptr(cs, fp)),
-           "tiny_mc_gpu_eval_step_math")
-    return t, sn, cs
+    _check(lib.tiny_mc_gpu_eval_step_math(math, _ptr(u, fp), u.size, _ptr(t, fp), _ptr(sn, fp), _ptr(cs, fp),
+                                          _ptr(sp, fp)), "tiny_mc_gpu_eval_step_math")
+    return t, sn, cs, sp
 
 
 def step_math_tables(math=MATH_HARDWARE):
-    """(t, sin, cos) of the walk for every uniform it can draw, indexed by
-    r >> 8 (2^24 entries each)."""
+    """(t, sin, cos, sin_theta) of the walk for every uniform it can draw,
+    indexed by r >> 8 (2^24 entries each)."""
     u = ((np.arange(1 << 24, dtype=np.float64) + 1.0) * 2.0 ** -24).astype(np.float32)
     return eval_step_math(u, math)
 
@@ -239,6 +243,23 @@ def check_sqrt(first_bits, count):
     n = ctypes.c_uint64()
     f = ctypes.c_uint32()
     _check(lib.tiny_mc_gpu_check_sqrt(first_bits, count, ctypes.byref(n), ctypes.byref(f)), "tiny_mc_gpu_check_sqrt")
+    return n.value, f.value
+
+
+CHECK_SQRT = 0
+CHECK_DIV_TENTH_SHORT = 1
+CHECK_DIV_TENTH = 2
+
+
+def check_short_op(op, first_bits, count):
+    """Mismatches of one of the walk's short forms (CHECK_*) against the
+    device IEEE operation over float bit patterns [first_bits, first_bits +
+    count): (count, lowest mismatching pattern)."""
+    lib = load()
+    n = ctypes.c_uint64()
+    f = ctypes.c_uint32()
+    _check(lib.tiny_mc_gpu_check_short_op(op, first_bits, count, ctypes.byref(n), ctypes.byref(f)),
+           "tiny_mc_gpu_check_short_op")
     return n.value, f.value
 
 

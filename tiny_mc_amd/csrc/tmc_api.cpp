@@ -433,7 +433,7 @@ namespace {
 
 /* Device buffers of a diagnostic call, freed on every path. */
 struct DevBufs {
-    void* p[4] = { nullptr, nullptr, nullptr, nullptr };
+    void* p[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
     ~DevBufs()
     {
         for (void* q : p)
@@ -468,9 +468,9 @@ TMC_EXPORT int tiny_mc_gpu_philox(const uint32_t* ctr, const uint32_t* key, uint
 }
 
 TMC_EXPORT int tiny_mc_gpu_eval_step_math(uint32_t math, const float* in, uint32_t n, float* step_out,
-                                          float* sin_out, float* cos_out)
+                                          float* sin_out, float* cos_out, float* polar_out)
 {
-    if (((!in || !step_out || !sin_out || !cos_out) && n) || math > TMC_MATH_OWN)
+    if (((!in || !step_out || !sin_out || !cos_out || !polar_out) && n) || math > TMC_MATH_OWN)
         return TMC_E_INVALID;
     std::lock_guard<std::mutex> lock(g_mutex);
     Context* c = nullptr;
@@ -479,16 +479,17 @@ TMC_EXPORT int tiny_mc_gpu_eval_step_math(uint32_t math, const float* in, uint32
         return rc;
     DevBufs b;
     const size_t bytes = 4 * (size_t)n;
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 5; ++i)
         if (hipMalloc(&b.p[i], bytes) != hipSuccess)
             return TMC_E_NOMEM;
     if (hipMemcpyAsync(b.p[0], in, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess
         || tmc::launch_eval_step_math(math, (const float*)b.p[0], n, (float*)b.p[1], (float*)b.p[2],
-                                      (float*)b.p[3], c->stream)
+                                      (float*)b.p[3], (float*)b.p[4], c->stream)
             != hipSuccess
         || hipMemcpyAsync(step_out, b.p[1], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
         || hipMemcpyAsync(sin_out, b.p[2], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
         || hipMemcpyAsync(cos_out, b.p[3], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipMemcpyAsync(polar_out, b.p[4], bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess
         || hipStreamSynchronize(c->stream) != hipSuccess)
         return TMC_E_HIP;
     return TMC_OK;
@@ -519,10 +520,11 @@ TMC_EXPORT int tiny_mc_gpu_eval_math(const float* in, uint32_t n, float* log_out
     return TMC_OK;
 }
 
-TMC_EXPORT int tiny_mc_gpu_check_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismatches,
-                                      uint32_t* first_mismatch)
+TMC_EXPORT int tiny_mc_gpu_check_short_op(uint32_t op, uint32_t first_bits, uint64_t count, uint64_t* mismatches,
+                                          uint32_t* first_mismatch)
 {
-    if (!mismatches || !first_mismatch || count > (1ull << 32) || (uint64_t)first_bits + count > (1ull << 32))
+    if (!mismatches || !first_mismatch || op > TMC_CHECK_DIV_TENTH || count > (1ull << 32)
+        || (uint64_t)first_bits + count > (1ull << 32))
         return TMC_E_INVALID;
     std::lock_guard<std::mutex> lock(g_mutex);
     Context* c = nullptr;
@@ -534,11 +536,17 @@ TMC_EXPORT int tiny_mc_gpu_check_sqrt(uint32_t first_bits, uint64_t count, uint6
     if (hipMalloc(&b.p[0], sizeof h) != hipSuccess)
         return TMC_E_NOMEM;
     if (hipMemcpyAsync(b.p[0], h, sizeof h, hipMemcpyHostToDevice, c->stream) != hipSuccess
-        || tmc::launch_check_sqrt(first_bits, count, (unsigned long long*)b.p[0], c->stream) != hipSuccess
+        || tmc::launch_check_short_op(op, first_bits, count, (unsigned long long*)b.p[0], c->stream) != hipSuccess
         || hipMemcpyAsync(h, b.p[0], sizeof h, hipMemcpyDeviceToHost, c->stream) != hipSuccess
         || hipStreamSynchronize(c->stream) != hipSuccess)
         return TMC_E_HIP;
     *mismatches = h[0];
     *first_mismatch = h[0] ? (uint32_t)h[1] : 0u;
     return TMC_OK;
+}
+
+TMC_EXPORT int tiny_mc_gpu_check_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismatches,
+                                      uint32_t* first_mismatch)
+{
+    return tiny_mc_gpu_check_short_op(TMC_CHECK_SQRT, first_bits, count, mismatches, first_mismatch);
 }

@@ -324,12 +324,14 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
     uint32_t poll_wait = 0; /* serving: steps before re-polling an unpublished ticket */
 
-    bool alive = false;
+    /* a lane holds a live photon while its weight is >= 0; -1 marks a lane
+     * without one (a comparison, so each ballot of it is one vector compare) */
     Photon ph;
     launch_photon(ph, 0, 0u, 0u);
+    ph.weight = -1;
 
     for (;;) {
-        uint64_t dead = __ballot(!alive);
+        uint64_t dead = __ballot(ph.weight < 0);
         if (dead != 0) {
             if (SERVE && poll_wait && dead != ~0ull) {
                 --poll_wait; /* live lanes keep walking; poll again later */
@@ -375,40 +377,44 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             if (avail) {
                 const uint32_t ndead = (uint32_t)__popcll(dead);
                 const uint32_t rank = mask_rank(dead);
-                if (!alive && rank < avail) {
+                if (ph.weight < 0 && rank < avail)
                     launch_photon(ph, st.first_photon + pool_next + rank, st.phys.key0, st.phys.key1);
-                    alive = true;
-                }
                 pool_next += avail < ndead ? avail : ndead;
             }
         }
-        const uint64_t live = __ballot(alive);
-        if (live == 0) {
-            if (queue_done && !have_pending)
-                break;
-            continue;
-        }
-        const uint64_t n_live = (uint64_t)__popcll(live);
-        wave_interactions += n_live;
-        absorbed += n_live;
-        if (!SERVE)
-            wave_idle += 64u - n_live;
+        const uint64_t live = __ballot(ph.weight >= 0);
+        if (live == 0 && queue_done && !have_pending)
+            break;
+        if (live != 0) {
+            const uint64_t n_live = (uint64_t)__popcll(live);
+            wave_interactions += n_live;
+            absorbed += n_live;
+            if (!SERVE)
+                wave_idle += 64u - n_live;
 
-        bool truncated = false;
-        if (alive) {
-            alive = step<SAMPLER, MATH>(ph, st.phys, lt);
-            if (alive && ph.k >= st.kcap) {
-                alive = false;
-                truncated = true;
+            /* every lane steps, only live ones deposit: a lane without a photon
+             * costs no extra vector cycles, and with no lane-masked region
+             * around the photon state the compiler keeps that state in place
+             * (a masked step made it copy the state in and out each step) */
+            const bool was_live = ph.weight >= 0;
+            const bool survives = step<SAMPLER, MATH>(ph, st.phys, lt, was_live);
+            ph.weight = was_live && survives ? ph.weight : -1.0f; /* stays -1 at albedo 0 */
+            /* photons that reached the absorption cap stop (rare) */
+            const uint64_t capped = __ballot(ph.k >= st.kcap) & __ballot(ph.weight >= 0);
+            if (capped != 0) {
+                wave_truncated += (uint64_t)__popcll(capped);
+                if (ph.k >= st.kcap)
+                    ph.weight = -1;
             }
-        }
-        wave_truncated += (uint64_t)__popcll(__ballot(truncated));
-        completed += (uint64_t)__popcll(live & __ballot(!alive));
+            completed += (uint64_t)__popcll(live & __ballot(ph.weight < 0));
 
-        if (++since_flush >= a.flush_interval) {
-            /* overflow guard only: completions are published on switch / idle */
-            since_flush = 0;
-            flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
+            /* wave-uniform; readfirstlane keeps it in a scalar register */
+            since_flush = __builtin_amdgcn_readfirstlane(since_flush + 1u);
+            if (since_flush >= a.flush_interval) {
+                /* overflow guard only: completions are published on switch / idle */
+                since_flush = 0;
+                flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
+            }
         }
     }
     if (cur_set != 0xFFFFFFFFu)

@@ -129,6 +129,24 @@ __device__ __forceinline__ void sincos_turn(uint32_t r, float* s_out, float* c_o
     }
 }
 
+/* Forward declaration: defined below. */
+__device__ __forceinline__ float sqrt_cr_walk(float x);
+
+/* sin(theta) = sqrt(1 - mu^2) of the analytic direction, mu = 2u - 1 on the
+ * uniform's 2^24-point grid (the radicand, rounded once by the FMA, is 0 or
+ * >= 2^-23).  TMC_MATH_HARDWARE takes the hardware square root (v_sqrt_f32,
+ * within one ulp): like -ln u and sin/cos it is then a function of the
+ * uniform's 24 bits, which a host program reproduces from a table of the
+ * device's values.  TMC_MATH_OWN rounds correctly. */
+template <uint32_t MATH>
+__device__ __forceinline__ float sin_polar(float mu)
+{
+    const float x = __builtin_fmaf(-mu, mu, 1.0f);
+    if (MATH == TMC_MATH_HARDWARE)
+        return __builtin_amdgcn_sqrtf(x);
+    return sqrt_cr_walk(x);
+}
+
 /* Correctly rounded square root (IEEE), the compiler's default lowering of
  * sqrtf under -fhip-fp32-correctly-rounded-divide-sqrt. */
 __device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
@@ -165,6 +183,28 @@ __device__ __forceinline__ float sqrt_radius(float r2)
     if (__builtin_expect(__ballot(r2 < TMC_SQRT_WALK_MIN) != 0, 0))
         return sqrt_rn(r2);
     return sqrt_cr_walk(r2);
+}
+
+/* w / 0.1f correctly rounded, as the product of w and the real constant
+ * C = 1 / 0.1f = 9.99999985...: fma(w, C_hi, w * C_lo) with C_hi = RN(C) = 10
+ * and C_lo = RN(C - C_hi) = -0x1.4p-23 (Brisebarre and Muller, "Correctly
+ * rounded multiplication by arbitrary precision constants", IEEE TC 2008).
+ * For this constant the form equals the IEEE quotient on every float with
+ * |w| >= 2^-100 (exhaustive device check, tiny_mc_gpu_check_short_op; the
+ * first failures lie below 2^-106, where w * C_lo loses bits to
+ * underflow).  One multiply and one FMA instead of the division sequence. */
+#define TMC_DIV_TENTH_MIN 0x1p-100f
+
+__device__ __forceinline__ float div_tenth_short(float w) { return __builtin_fmaf(w, 10.0f, w * -0x1.4p-23f); }
+
+/* w / 0.1f for any w: the short form unless some lane of the wave holds a
+ * weight of magnitude below 2^-100 (reachable only with albedos below
+ * ~1e-27). */
+__device__ __forceinline__ float div_tenth(float w)
+{
+    if (__builtin_expect(__ballot(__builtin_fabsf(w) < TMC_DIV_TENTH_MIN) != 0, 0))
+        return w / 0.1f;
+    return div_tenth_short(w);
 }
 
 /* Fixed-point images of one deposit: w * 2^33 and (w*w) * 2^43, truncated.

@@ -11,7 +11,7 @@
  * 0xCD9E8D57, Weyl key increments 0x9E3779B9 and 0xBB67AE85.
  *
  * On gfx950 each round is two v_mad_u64_u32 (hi and lo of a 32x32 product in
- * one instruction) plus XORs; the key schedule depends only on the seed, so
+ * one instruction) plus two three-input XORs; the key schedule depends only on the seed, so
  * within a wave it lives in scalar registers.
  */
 #ifndef TMC_PHILOX_H
@@ -30,6 +30,18 @@
 #define TMC_PHILOX_W0 0x9E3779B9u
 #define TMC_PHILOX_W1 0xBB67AE85u
 
+/* a ^ b ^ c.  In device code one v_bitop3_b32 (truth table 0x96, the
+ * three-input exclusive or), which halves the XORs of a round: the compiler
+ * emits two v_xor_b32 for the plain expression. */
+TMC_HD uint32_t tmc_xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 typedef struct tmc_u32x4 {
     uint32_t x0, x1, x2, x3;
 } tmc_u32x4;
@@ -42,8 +54,8 @@ TMC_HD tmc_u32x4 tmc_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32
     for (int round = 0; round < 10; ++round) {
         const uint64_t p0 = (uint64_t)c0 * TMC_PHILOX_M0;
         const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = tmc_xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = tmc_xor3((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -85,8 +97,8 @@ TMC_HD tmc_u32x4 tmc_philox4x32_10_block0(uint32_t k, tmc_philox_photon pp, uint
     for (int round = 1; round < 10; ++round) {
         const uint64_t p0 = (uint64_t)c0 * TMC_PHILOX_M0;
         const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = tmc_xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = tmc_xor3((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;

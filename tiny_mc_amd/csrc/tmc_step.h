@@ -53,12 +53,19 @@ struct LaneTally {
     unsigned long long* hot2;
 };
 
+/* The roulette kills a photon when u > 0.1f (photon.c:46).  For
+ * u = ((r >> 8) + 1) 2^-24 that is (r >> 8) + 1 > 0.1f * 2^24 = 1677721.625,
+ * i.e. r >= 1677721 * 2^8: one integer compare on the Philox word. */
+#define TMC_ROULETTE_KILL_R 0x19999900u
+
 /* Walks one absorption event of p, adding its deposit to the fixed-point
- * shell sums in LDS.  Returns false when the roulette kills the photon.
- * MATH (TMC_MATH_*) selects how -ln u and sin/cos(2 pi u') are evaluated;
- * everything else is the same arithmetic in both modes. */
+ * shell sums in LDS unless deposit is false.  Returns false when the
+ * roulette kills the photon.
+ * MATH (TMC_MATH_*) selects how -ln u, sin/cos(2 pi u') and the analytic
+ * sampler's sin(theta) are evaluated; everything else is the same
+ * arithmetic in both modes. */
 template <uint32_t SAMPLER, uint32_t MATH>
-__device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt)
+__device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt, bool deposit = true)
 {
     const tmc_u32x4 r = tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1); /* block (k, 0) */
 
@@ -72,8 +79,10 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
     const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
     const float f = sqrt_radius(r2) * ph.spm;
     const bool inner = f < ph.shell_max; /* else the overflow shell, ph.last_shell */
-    atomicAdd(inner ? lt.w1 + (uint32_t)f : lt.hot1, (unsigned long long)fx_heat(p.weight));
-    atomicAdd(inner ? lt.w2 + (uint32_t)f : lt.hot2, (unsigned long long)fx_heat2(p.weight));
+    if (deposit) {
+        atomicAdd(inner ? lt.w1 + (uint32_t)f : lt.hot1, (unsigned long long)fx_heat(p.weight));
+        atomicAdd(inner ? lt.w2 + (uint32_t)f : lt.hot2, (unsigned long long)fx_heat2(p.weight));
+    }
     p.weight = p.weight * ph.albedo;
 
     /* new direction (photon.c:34-43) */
@@ -104,7 +113,7 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
         p.w = xi2 * s;
     } else {
         const float mu = __builtin_fmaf(tmc_u24(r.x1), 0x1p-23f, -1.0f); /* = 2 u - 1, both exact */
-        const float s = sqrt_cr_walk(__builtin_fmaf(-mu, mu, 1.0f)); /* 0 or >= 2^-23 */
+        const float s = sin_polar<MATH>(mu);
         float sn, cs;
         sincos_turn<MATH>(r.x2, &sn, &cs);
         p.u = mu;
@@ -115,9 +124,9 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
 
     /* roulette (photon.c:45-49) */
     if (p.weight < 0.001f) {
-        if (tmc_u01(r.x3) > 0.1f)
+        if (r.x3 >= TMC_ROULETTE_KILL_R) /* u > 0.1f */
             return false;
-        p.weight = p.weight / 0.1f;
+        p.weight = div_tenth(p.weight); /* = p.weight / 0.1f */
     }
     return true;
 }

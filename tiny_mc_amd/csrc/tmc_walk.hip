@@ -111,13 +111,16 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
     uint64_t wave_interactions = 0, wave_truncated = 0;
     uint32_t since_flush = 0;
 
-    /* lane state: one photon (photon.c:11-18) */
-    bool alive = false;
+    /* lane state: one photon (photon.c:11-18).  A lane holds a live photon
+     * while its weight is >= 0 (a live weight can be 0, at albedo 0); -1
+     * marks a lane without one.  Keeping the flag a comparison lets every
+     * ballot of it be one vector compare. */
     PhotonT ph;
     launch_photon(ph, 0, phys.key0, phys.key1);
+    ph.weight = -1;
 
     for (;;) {
-        const uint64_t dead = __ballot(!alive);
+        const uint64_t dead = __ballot(ph.weight < 0);
         if (dead != 0 && !exhausted) {
             const uint32_t ndead = (uint32_t)__popcll(dead);
             const uint32_t rank = mask_rank(dead);
@@ -135,7 +138,7 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
                     exhausted = true;
                 }
             }
-            if (!alive) {
+            if (ph.weight < 0) {
                 bool got = false;
                 uint64_t idx = 0;
                 if (rank < avail) {
@@ -145,10 +148,8 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
                     idx = nb + (rank - avail);
                     got = true;
                 }
-                if (got) {
+                if (got)
                     launch_photon(ph, a.first_photon + idx, phys.key0, phys.key1);
-                    alive = true;
-                }
             }
             if (avail >= ndead) {
                 pool_next += ndead;
@@ -158,22 +159,25 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
                 pool_end = ne;
             }
         }
-        const uint64_t live = __ballot(alive);
+        const uint64_t live = __ballot(ph.weight >= 0);
         if (live == 0)
             break;
         wave_interactions += (uint64_t)__popcll(live);
 
-        bool truncated = false;
-        if (alive) {
-            alive = step_of<SAMPLER, VARIANT>(ph, phys, lt);
-            if (alive && ph.k >= a.kcap) {
-                alive = false;
-                truncated = true;
-            }
+        if (ph.weight >= 0 && !step_of<SAMPLER, VARIANT>(ph, phys, lt))
+            ph.weight = -1;
+        /* photons that reached the absorption cap stop (rare: a uniform
+         * branch around the bookkeeping) */
+        const uint64_t capped = __ballot(ph.k >= a.kcap) & __ballot(ph.weight >= 0);
+        if (capped != 0) {
+            wave_truncated += (uint64_t)__popcll(capped);
+            if (ph.k >= a.kcap)
+                ph.weight = -1;
         }
-        wave_truncated += (uint64_t)__popcll(__ballot(truncated));
 
-        if (++since_flush >= a.flush_interval) {
+        /* wave-uniform; readfirstlane keeps it in a scalar register */
+        since_flush = __builtin_amdgcn_readfirstlane(since_flush + 1u);
+        if (since_flush >= a.flush_interval) {
             since_flush = 0;
             flush_tallies(lw, lw2, hot1, hot2, a.tally, a.shells, lane);
         }
@@ -274,17 +278,29 @@ __global__ void eval_math_kernel(const float* in, uint32_t n, float* lg, float* 
     cs[i] = c;
 }
 
-/* Compares sqrt_cr_walk with the IEEE square root on the floats with bit
+/* The walk's short forms against the IEEE operation they replace. */
+template <uint32_t OP>
+__device__ __forceinline__ bool short_op_matches(float x)
+{
+    if (OP == TMC_CHECK_SQRT)
+        return __float_as_uint(sqrt_cr_walk(x)) == __float_as_uint(sqrt_rn(x));
+    if (OP == TMC_CHECK_DIV_TENTH_SHORT)
+        return __float_as_uint(div_tenth_short(x)) == __float_as_uint(x / 0.1f);
+    return __float_as_uint(div_tenth(x)) == __float_as_uint(x / 0.1f);
+}
+
+/* Compares a short form with the IEEE operation on the floats with bit
  * patterns [first, first + count): out[0] += mismatches, out[1] = min of the
  * mismatching patterns. */
-__global__ void check_sqrt_kernel(uint32_t first, uint64_t count, unsigned long long* out)
+template <uint32_t OP>
+__global__ void check_short_op_kernel(uint32_t first, uint64_t count, unsigned long long* out)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     unsigned long long bad = 0, lowest = ~0ull;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         const uint32_t bits = first + (uint32_t)i;
         const float x = __uint_as_float(bits);
-        if (__float_as_uint(sqrt_cr_walk(x)) != __float_as_uint(sqrt_rn(x))) {
+        if (!short_op_matches<OP>(x)) {
             ++bad;
             lowest = bits < lowest ? bits : lowest;
         }
@@ -295,18 +311,27 @@ __global__ void check_sqrt_kernel(uint32_t first, uint64_t count, unsigned long 
     }
 }
 
-hipError_t launch_check_sqrt(uint32_t first, uint64_t count, unsigned long long* out, hipStream_t stream)
+hipError_t launch_check_short_op(uint32_t op, uint32_t first, uint64_t count, unsigned long long* out,
+                                 hipStream_t stream)
 {
     if (count == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(check_sqrt_kernel, dim3(8192), dim3(256), 0, stream, first, count, out);
+    if (op == TMC_CHECK_SQRT)
+        hipLaunchKernelGGL(check_short_op_kernel<TMC_CHECK_SQRT>, dim3(8192), dim3(256), 0, stream, first, count,
+                           out);
+    else if (op == TMC_CHECK_DIV_TENTH_SHORT)
+        hipLaunchKernelGGL(check_short_op_kernel<TMC_CHECK_DIV_TENTH_SHORT>, dim3(8192), dim3(256), 0, stream,
+                           first, count, out);
+    else
+        hipLaunchKernelGGL(check_short_op_kernel<TMC_CHECK_DIV_TENTH>, dim3(8192), dim3(256), 0, stream, first,
+                           count, out);
     return hipGetLastError();
 }
 
 /* The step length and sin/cos(2 pi u) of a binary32 walk in math mode MATH
  * on inputs u = in[i] (each of the form ((r >> 8) + 1) 2^-24). */
 template <uint32_t MATH>
-__global__ void eval_step_math_kernel(const float* in, uint32_t n, float* t, float* sn, float* cs)
+__global__ void eval_step_math_kernel(const float* in, uint32_t n, float* t, float* sn, float* cs, float* sp)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -318,19 +343,21 @@ __global__ void eval_step_math_kernel(const float* in, uint32_t n, float* t, flo
     sincos_turn<MATH>(r, &s, &c);
     sn[i] = s;
     cs[i] = c;
+    /* the analytic sampler's sin(theta) for mu = 2u - 1 */
+    sp[i] = sin_polar<MATH>(__builtin_fmaf(tmc_u24(r), 0x1p-23f, -1.0f));
 }
 
 hipError_t launch_eval_step_math(uint32_t math, const float* in, uint32_t n, float* t, float* sn, float* cs,
-                                 hipStream_t stream)
+                                 float* sp, hipStream_t stream)
 {
     if (n == 0)
         return hipSuccess;
     if (math == TMC_MATH_OWN)
         hipLaunchKernelGGL(eval_step_math_kernel<TMC_MATH_OWN>, dim3((n + 255) / 256), dim3(256), 0, stream, in, n,
-                           t, sn, cs);
+                           t, sn, cs, sp);
     else
         hipLaunchKernelGGL(eval_step_math_kernel<TMC_MATH_HARDWARE>, dim3((n + 255) / 256), dim3(256), 0, stream,
-                           in, n, t, sn, cs);
+                           in, n, t, sn, cs, sp);
     return hipGetLastError();
 }
 

@@ -41,7 +41,8 @@
 namespace tmc {
 
 /* Copies of the overflow shell's sums in each wave's slot (lane & 15 picks
- * one), after the slot's shell copies. */
+ * one), after the slot's shell copies.  Shells and copies alike are
+ * (w image, w*w image) pairs of adjacent words. */
 #define TMC_SLOT_HOT 16u
 
 /* 64-bit words of one wave's LDS slot. */
@@ -66,17 +67,17 @@ __device__ __forceinline__ void flush_slot(unsigned long long* slot, uint32_t sh
         if (s < shells) {
             for (uint32_t c = 0; c < copies; ++c) {
                 unsigned long long* lw = slot + (size_t)c * 2 * shells_max;
-                a += lw[s];
-                b += lw[shells_max + s];
-                lw[s] = 0;
-                lw[shells_max + s] = 0;
+                a += lw[2 * s];
+                b += lw[2 * s + 1];
+                lw[2 * s] = 0;
+                lw[2 * s + 1] = 0;
             }
         } else {
             const uint32_t h = s - shells;
-            a = hot[h];
-            b = hot[TMC_SLOT_HOT + h];
-            hot[h] = 0;
-            hot[TMC_SLOT_HOT + h] = 0;
+            a = hot[2 * h];
+            b = hot[2 * h + 1];
+            hot[2 * h] = 0;
+            hot[2 * h + 1] = 0;
             shell = shells - 1;
         }
         if (SERVE) {
@@ -309,8 +310,8 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     for (uint32_t i = lane; i < slot_words(a.shells_max, a.copies); i += 64)
         slot[i] = 0;
     unsigned long long* lw = slot + (a.copies == 2 && lane >= 32 ? 2 * a.shells_max : 0);
-    unsigned long long* hot = slot + (size_t)a.copies * 2 * a.shells_max + (lane & (TMC_SLOT_HOT - 1));
-    const LaneTally lt = { lw, lw + a.shells_max, hot, hot + TMC_SLOT_HOT };
+    unsigned long long* hot = slot + (size_t)a.copies * 2 * a.shells_max + 2 * (lane & (TMC_SLOT_HOT - 1));
+    const LaneTally lt = { lw, hot };
     const uint32_t slot_mask = SERVE ? (1u << TMC_SERVE_SLOT_BITS) - 1u : 0xFFFFFFFFu;
 
     SetState st;

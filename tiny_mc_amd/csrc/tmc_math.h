@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include "../../include/tiny_mc_gpu.h"
+#include "tmc_philox.h"
 
 namespace tmc {
 
@@ -113,14 +114,14 @@ template <uint32_t MATH>
 __device__ __forceinline__ float step_length(uint32_t r)
 {
     if (MATH == TMC_MATH_HARDWARE)
-        return __builtin_amdgcn_logf((float)((r >> 8) + 1u) * 0x1p-24f) * -0x1.62e430p-1f;
+        return __builtin_amdgcn_logf(tmc_u01_fma(r)) * -0x1.62e430p-1f;
     return -log32_u24((float)((r >> 8) + 1u)); /* = -log32(u) */
 }
 
 template <uint32_t MATH>
 __device__ __forceinline__ void sincos_turn(uint32_t r, float* s_out, float* c_out)
 {
-    const float a = (float)((r >> 8) + 1u) * 0x1p-24f;
+    const float a = tmc_u01_fma(r);
     if (MATH == TMC_MATH_HARDWARE) {
         *s_out = __builtin_amdgcn_sinf(a);
         *c_out = __builtin_amdgcn_cosf(a);
@@ -215,18 +216,22 @@ __device__ __forceinline__ float div_tenth(float w)
 #define TMC_HEAT_FRAC_BITS 33
 #define TMC_HEAT2_FRAC_BITS 43
 
-/* floor(x) as a 64-bit integer for 0 <= x < 2^44: the high word floor(x / 2^32)
- * and the low word x - hi * 2^32 are exact in binary32 (x >= 2^32 is an
- * integer), and the low word's conversion truncates. */
-__device__ __forceinline__ uint64_t floor_u64_small(float x)
+/* floor(y * 2^32) as a 64-bit integer for 0 <= y < 2^24: the high word is
+ * trunc(y) and the low word trunc(fract(y) * 2^32); fract(y) = y - floor(y)
+ * (v_fract_f32) and its scaling by 2^32 are exact, so the pair is the
+ * truncation of the real y * 2^32.  Three instructions where the generic
+ * float -> u64 conversion takes many more. */
+__device__ __forceinline__ uint64_t fixed_u64(float y)
 {
-    const float hi = __builtin_floorf(x * 0x1p-32f);
-    const float lo = __builtin_fmaf(hi, -0x1p32f, x);
-    return ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo;
+    const uint32_t hi = (uint32_t)y;
+    const uint32_t lo = (uint32_t)(__builtin_amdgcn_fractf(y) * 0x1p32f);
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
 }
 
-__device__ __forceinline__ uint64_t fx_heat(float w) { return floor_u64_small(w * 0x1p33f); }
-__device__ __forceinline__ uint64_t fx_heat2(float w) { return floor_u64_small((w * w) * 0x1p43f); }
+/* w * 2^33 = (2 w) * 2^32 and (w*w) * 2^43 = ((w*w) * 2^11) * 2^32; the
+ * power-of-two scalings are exact. */
+__device__ __forceinline__ uint64_t fx_heat(float w) { return fixed_u64(w * 2.0f); }
+__device__ __forceinline__ uint64_t fx_heat2(float w) { return fixed_u64((w * w) * 0x1p11f); }
 
 } // namespace tmc
 

@@ -124,4 +124,19 @@ TMC_HD float tmc_u24(uint32_t r)
     return (float)((r >> 8) + 1u);
 }
 
+/* tmc_u01(r) from one conversion and one FMA: (r >> 8) 2^-24 + 2^-24 is
+ * exact in binary32 (at most 24 significant bits), so the FMA's single
+ * rounding returns the same float. */
+TMC_HD float tmc_u01_fma(uint32_t r)
+{
+    return __builtin_fmaf((float)(r >> 8), 0x1p-24f, 0x1p-24f);
+}
+
+/* 2u - 1 for the same uniform: (r >> 8) 2^-23 + (2^-23 - 1), an exact
+ * multiple of 2^-23 in (-1, 1], equal to fma(tmc_u24(r), 2^-23, -1). */
+TMC_HD float tmc_u11(uint32_t r)
+{
+    return __builtin_fmaf((float)(r >> 8), 0x1p-23f, 0x1p-23f - 1.0f);
+}
+
 #endif

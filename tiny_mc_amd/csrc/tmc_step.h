@@ -41,17 +41,26 @@ __device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon, uint32
     p.pp = tmc_philox_photon_init(p.plo, p.phi, key0, key1);
 }
 
-/* LDS words one lane deposits into: the shell sums (w and w*w images) and
- * the lane's own copy of the overflow shell's sums.  A long walk ends up in
- * the overflow shell (the last one) most of the time, so there the lanes
- * of a wave would all hit one word; each lane group has its own copy
- * instead, and the copies are added up when the sums are flushed. */
+/* LDS words one lane deposits into.  Each shell's two sums (w image, w*w
+ * image) are adjacent words, 2 s and 2 s + 1, so one address serves both
+ * atomics (the second takes the instruction's offset field).  A long walk
+ * ends up in the overflow shell (the last one) most of the time, where the
+ * lanes of a wave would all hit one pair; each lane group has its own copy
+ * of that pair instead, laid out the same way, and the copies are added up
+ * when the sums are flushed. */
 struct LaneTally {
-    unsigned long long* w1;
-    unsigned long long* w2;
-    unsigned long long* hot1;
-    unsigned long long* hot2;
+    unsigned long long* shell; /* pair of shell s at shell[2 s], shell[2 s + 1] */
+    unsigned long long* hot; /* this lane's copy of the overflow shell's pair */
 };
+
+/* Adds one deposit's images to shell s (inner) or to the overflow pair. */
+__device__ __forceinline__ void tally_add(const LaneTally& lt, bool inner, uint32_t s, unsigned long long a,
+                                          unsigned long long b)
+{
+    unsigned long long* p = inner ? lt.shell + 2u * s : lt.hot;
+    atomicAdd(p, a);
+    atomicAdd(p + 1, b);
+}
 
 /* The roulette kills a photon when u > 0.1f (photon.c:46).  For
  * u = ((r >> 8) + 1) 2^-24 that is (r >> 8) + 1 > 0.1f * 2^24 = 1677721.625,
@@ -79,10 +88,8 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
     const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
     const float f = sqrt_radius(r2) * ph.spm;
     const bool inner = f < ph.shell_max; /* else the overflow shell, ph.last_shell */
-    if (deposit) {
-        atomicAdd(inner ? lt.w1 + (uint32_t)f : lt.hot1, (unsigned long long)fx_heat(p.weight));
-        atomicAdd(inner ? lt.w2 + (uint32_t)f : lt.hot2, (unsigned long long)fx_heat2(p.weight));
-    }
+    if (deposit)
+        tally_add(lt, inner, (uint32_t)f, fx_heat(p.weight), fx_heat2(p.weight));
     p.weight = p.weight * ph.albedo;
 
     /* new direction (photon.c:34-43) */
@@ -92,8 +99,8 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
         float xi1, xi2, tt;
         uint32_t att = 0;
         for (;;) {
-            xi1 = __builtin_fmaf(tmc_u24(w0), 0x1p-23f, -1.0f);
-            xi2 = __builtin_fmaf(tmc_u24(w1), 0x1p-23f, -1.0f);
+            xi1 = tmc_u11(w0);
+            xi2 = tmc_u11(w1);
             tt = __builtin_fmaf(xi2, xi2, xi1 * xi1);
             ++att;
             if ((tt <= 1.0f && tt > 0.0f) || att >= 64u)
@@ -112,7 +119,7 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
         p.v = xi1 * s;
         p.w = xi2 * s;
     } else {
-        const float mu = __builtin_fmaf(tmc_u24(r.x1), 0x1p-23f, -1.0f); /* = 2 u - 1, both exact */
+        const float mu = tmc_u11(r.x1); /* = 2 u - 1, exact */
         const float s = sin_polar<MATH>(mu);
         float sn, cs;
         sincos_turn<MATH>(r.x2, &sn, &cs);

@@ -41,16 +41,17 @@ namespace tmc {
  * overflow shell's copies of both. */
 __host__ __device__ __forceinline__ size_t walk_lds_words(uint32_t shells) { return 2u * shells + 2u * TMC_WALK_HOT; }
 
-/* Moves this workgroup's LDS tallies into the global 128-bit sums.  Safe
- * while other waves of the workgroup keep adding: each entry is taken by an
- * atomic swap with zero, so every addition is counted exactly once. */
-__device__ __forceinline__ void flush_tallies(unsigned long long* lw, unsigned long long* lw2,
-                                              unsigned long long* hot1, unsigned long long* hot2,
+/* Moves this workgroup's LDS tallies (shell pairs, then the overflow
+ * shell's copies, each a (w image, w*w image) pair of words) into the global
+ * 128-bit sums.  Safe while other waves of the workgroup keep adding: each
+ * entry is taken by an atomic swap with zero, so every addition is counted
+ * exactly once. */
+__device__ __forceinline__ void flush_tallies(unsigned long long* lds, unsigned long long* hot,
                                               unsigned long long* g, uint32_t shells, uint32_t lane)
 {
     for (uint32_t s = lane; s < shells; s += 64) {
-        const unsigned long long a = atomicExch(&lw[s], 0ull);
-        const unsigned long long b = atomicExch(&lw2[s], 0ull);
+        const unsigned long long a = atomicExch(&lds[2 * s], 0ull);
+        const unsigned long long b = atomicExch(&lds[2 * s + 1], 0ull);
         if (a)
             add128(g + 2 * s, a);
         if (b)
@@ -58,8 +59,8 @@ __device__ __forceinline__ void flush_tallies(unsigned long long* lw, unsigned l
     }
     if (lane < TMC_WALK_HOT) {
         const uint32_t last = shells - 1;
-        const unsigned long long a = atomicExch(&hot1[lane], 0ull);
-        const unsigned long long b = atomicExch(&hot2[lane], 0ull);
+        const unsigned long long a = atomicExch(&hot[2 * lane], 0ull);
+        const unsigned long long b = atomicExch(&hot[2 * lane + 1], 0ull);
         if (a)
             add128(g + 2 * last, a);
         if (b)
@@ -94,16 +95,13 @@ template <class PhotonT, class PhysicsT, uint32_t SAMPLER, uint32_t VARIANT>
 __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phys)
 {
     extern __shared__ unsigned long long lds_tally[];
-    unsigned long long* lw = lds_tally;
-    unsigned long long* lw2 = lds_tally + a.shells;
-    unsigned long long* hot1 = lds_tally + 2 * a.shells;
-    unsigned long long* hot2 = hot1 + TMC_WALK_HOT;
+    unsigned long long* hot = lds_tally + 2 * a.shells;
     for (uint32_t i = threadIdx.x; i < walk_lds_words(a.shells); i += blockDim.x)
         lds_tally[i] = 0;
     __syncthreads();
 
     const uint32_t lane = __lane_id();
-    const LaneTally lt = { lw, lw2, hot1 + (lane & (TMC_WALK_HOT - 1)), hot2 + (lane & (TMC_WALK_HOT - 1)) };
+    const LaneTally lt = { lds_tally, hot + 2 * (lane & (TMC_WALK_HOT - 1)) };
 
     /* wave-uniform pool of claimed photon offsets [pool_next, pool_end) */
     uint64_t pool_next = 0, pool_end = 0;
@@ -179,10 +177,10 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
         since_flush = __builtin_amdgcn_readfirstlane(since_flush + 1u);
         if (since_flush >= a.flush_interval) {
             since_flush = 0;
-            flush_tallies(lw, lw2, hot1, hot2, a.tally, a.shells, lane);
+            flush_tallies(lds_tally, hot, a.tally, a.shells, lane);
         }
     }
-    flush_tallies(lw, lw2, hot1, hot2, a.tally, a.shells, lane);
+    flush_tallies(lds_tally, hot, a.tally, a.shells, lane);
     if (lane == 0) {
         atomicAdd(a.counters + 1, (unsigned long long)wave_interactions);
         if (wave_truncated)
@@ -344,7 +342,7 @@ __global__ void eval_step_math_kernel(const float* in, uint32_t n, float* t, flo
     sn[i] = s;
     cs[i] = c;
     /* the analytic sampler's sin(theta) for mu = 2u - 1 */
-    sp[i] = sin_polar<MATH>(__builtin_fmaf(tmc_u24(r), 0x1p-23f, -1.0f));
+    sp[i] = sin_polar<MATH>(tmc_u11(r));
 }
 
 hipError_t launch_eval_step_math(uint32_t math, const float* in, uint32_t n, float* t, float* sn, float* cs,

@@ -176,6 +176,16 @@ int tiny_mc_gpu_check_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismat
 int tiny_mc_gpu_check_short_op(uint32_t op, uint32_t first_bits, uint64_t count, uint64_t* mismatches,
                                uint32_t* first_mismatch);
 
+/* Diagnostic: the binary32 walk takes a deposit's shell from the hardware
+ * square root where that is certain to give the shell of the correctly
+ * rounded one (photon.c:26-29).  Checks that claim for the physics p on
+ * every squared radius whose bit pattern lies in [first_bits, first_bits +
+ * count): violations (radii marked certain whose shell or overflow side
+ * differs) with the lowest such pattern, and the number of radii not marked
+ * certain (the walk recomputes those with the IEEE square root). */
+int tiny_mc_gpu_check_shell_index(const tmc_params* p, uint32_t first_bits, uint64_t count, uint64_t* violations,
+                                  uint32_t* first_violation, uint64_t* unsure);
+
 /* ------------------------------------------------------------------------
  * Batched sweep: many parameter sets in one persistent launch.
  *

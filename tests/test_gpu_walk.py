@@ -148,6 +148,23 @@ def test_short_sqrt_exhaustive(tmc):
     assert tmc.check_sqrt(lo, hi - lo + 1) == (0, 0)
 
 
+@pytest.mark.parametrize("phys", [dict(), dict(mu_a=10.0, mu_s=10.0, shells=51, microns_per_shell=100.0),
+                                  dict(mu_a=0.5, mu_s=50.0, shells=201, microns_per_shell=20.0),
+                                  dict(mu_a=0.01, mu_s=100.0, shells=401, microns_per_shell=10.0),
+                                  dict(shells=8192, microns_per_shell=1.0)])
+def test_shell_index_claim_exhaustive(tmc, phys):
+    """Where the walk takes a deposit's shell from the hardware square root,
+    the shell and the overflow side are those of the correctly rounded
+    radius, on every non-negative float squared radius (+inf included);
+    the other radii are recomputed with the IEEE square root."""
+    inf = int(np.float32(np.inf).view(np.uint32))
+    bad, first, unsure = tmc.check_shell_index(tmc.Params(**phys), 0, inf + 1)
+    assert (bad, first) == (0, 0)
+    # the fallback covers the tiny radii (about half of all float patterns
+    # lie below 2^-1) and the neighbourhoods of the shell boundaries
+    assert 0 < unsure < inf
+
+
 def test_roulette_division_exhaustive(tmc):
     """The roulette's weight update w / 0.1f as fma(w, 10, w * C_lo) equals
     the IEEE quotient on every float of magnitude from 2^-100 to the largest

@@ -33,6 +33,7 @@ from ._lib import (  # noqa: F401
     TmcError,
     check_sqrt,
     check_short_op,
+    check_shell_index,
     CHECK_SQRT,
     CHECK_DIV_TENTH_SHORT,
     CHECK_DIV_TENTH,

@@ -145,10 +145,12 @@ def load():
     lib.tiny_mc_gpu_check_sqrt.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u64p, P(ctypes.c_uint32)]
     lib.tiny_mc_gpu_check_short_op.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, u64p,
                                                P(ctypes.c_uint32)]
+    lib.tiny_mc_gpu_check_shell_index.argtypes = [P(_Params), ctypes.c_uint32, ctypes.c_uint64, u64p,
+                                                  P(ctypes.c_uint32), u64p]
     lib.tiny_mc_gpu_eval_step_math.argtypes = [ctypes.c_uint32, fp, ctypes.c_uint32, fp, fp, fp, fp]
     for name in ("tiny_mc_gpu_run", "tiny_mc_gpu_run_ex", "tiny_mc_gpu_tally_to_heat", "tiny_mc_gpu_init",
                  "tiny_mc_gpu_device_info", "tiny_mc_gpu_philox", "tiny_mc_gpu_eval_math", "tiny_mc_gpu_check_sqrt",
-                 "tiny_mc_gpu_eval_step_math", "tiny_mc_gpu_check_short_op"):
+                 "tiny_mc_gpu_eval_step_math", "tiny_mc_gpu_check_short_op", "tiny_mc_gpu_check_shell_index"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -249,6 +251,20 @@ def check_sqrt(first_bits, count):
 CHECK_SQRT = 0
 CHECK_DIV_TENTH_SHORT = 1
 CHECK_DIV_TENTH = 2
+
+
+def check_shell_index(params, first_bits, count):
+    """The walk's short shell computation for `params` on squared radii with
+    float bit patterns [first_bits, first_bits + count): (violations,
+    lowest violating pattern, radii recomputed with the IEEE sqrt)."""
+    lib = load()
+    n = ctypes.c_uint64()
+    f = ctypes.c_uint32()
+    u = ctypes.c_uint64()
+    p = params._c()
+    _check(lib.tiny_mc_gpu_check_shell_index(ctypes.byref(p), first_bits, count, ctypes.byref(n), ctypes.byref(f),
+                                             ctypes.byref(u)), "tiny_mc_gpu_check_shell_index")
+    return n.value, f.value, u.value
 
 
 def check_short_op(op, first_bits, count):

@@ -545,6 +545,37 @@ TMC_EXPORT int tiny_mc_gpu_check_short_op(uint32_t op, uint32_t first_bits, uint
     return TMC_OK;
 }
 
+TMC_EXPORT int tiny_mc_gpu_check_shell_index(const tmc_params* p, uint32_t first_bits, uint64_t count,
+                                             uint64_t* violations, uint32_t* first_violation, uint64_t* unsure)
+{
+    if (!violations || !first_violation || !unsure || count > (1ull << 32)
+        || (uint64_t)first_bits + count > (1ull << 32))
+        return TMC_E_INVALID;
+    int rc = tmc::check_params(p);
+    if (rc != TMC_OK)
+        return rc;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    Context* c = nullptr;
+    rc = tmc::acquire_context(&c);
+    if (rc != TMC_OK)
+        return rc;
+    DevBufs b;
+    unsigned long long h[3] = { 0ull, ~0ull, 0ull };
+    if (hipMalloc(&b.p[0], sizeof h) != hipSuccess)
+        return TMC_E_NOMEM;
+    if (hipMemcpyAsync(b.p[0], h, sizeof h, hipMemcpyHostToDevice, c->stream) != hipSuccess
+        || tmc::launch_check_shell(tmc::derived_spm(p), (float)(p->shells - 1), first_bits, count,
+                                   (unsigned long long*)b.p[0], c->stream)
+            != hipSuccess
+        || hipMemcpyAsync(h, b.p[0], sizeof h, hipMemcpyDeviceToHost, c->stream) != hipSuccess
+        || hipStreamSynchronize(c->stream) != hipSuccess)
+        return TMC_E_HIP;
+    *violations = h[0];
+    *first_violation = h[0] ? (uint32_t)h[1] : 0u;
+    *unsure = h[2];
+    return TMC_OK;
+}
+
 TMC_EXPORT int tiny_mc_gpu_check_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismatches,
                                       uint32_t* first_mismatch)
 {

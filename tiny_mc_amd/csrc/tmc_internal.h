@@ -43,6 +43,8 @@ hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t math, u
                           int* wg_per_cu);
 
 hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out, hipStream_t stream);
+hipError_t launch_check_shell(float spm, float shell_max, uint32_t first, uint64_t count, unsigned long long* out,
+                              hipStream_t stream);
 hipError_t launch_check_short_op(uint32_t op, uint32_t first, uint64_t count, unsigned long long* out,
                                  hipStream_t stream);
 hipError_t launch_eval_step_math(uint32_t math, const float* in, uint32_t n, float* t, float* sn, float* cs,

@@ -27,7 +27,52 @@ struct Physics {
     float shell_max; /* (float)(shells - 1) */
     uint32_t last_shell; /* shells - 1 */
     uint32_t key0, key1; /* Philox key = 64-bit seed */
+    float far_min; /* shell_max + 1, set by set_shell_bounds */
+    float clear_half; /* 0.5 - eps, set by set_shell_bounds */
 };
+
+/* Bounds of shell_index for the set's shell_max: a shell coordinate computed
+ * from the hardware square root is within 3 ulp of the exact one (below), so
+ * it has the exact floor if it lies more than eps = 8 ulp from every integer,
+ * with the ulp of the floats just below 2^e > shell_max + 2; at or beyond
+ * shell_max + 1 it is in the overflow shell either way. */
+__device__ __forceinline__ void set_shell_bounds(Physics& ph)
+{
+    const int e = __builtin_amdgcn_frexp_expf(ph.shell_max + 2.0f); /* shell_max + 2 < 2^e */
+    /* wave-uniform: keep the bounds in scalar registers */
+    ph.far_min = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(ph.shell_max + 1.0f)));
+    ph.clear_half =
+        __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(0.5f - __builtin_ldexpf(1.0f, e - 21))));
+}
+
+/* The short shell coordinate f' = RN(v_sqrt(r2) * spm) and whether its floor
+ * and its side of shell_max are certainly those of the exact coordinate. */
+__device__ __forceinline__ float shell_coord_short(float r2, const Physics& ph, bool* sure)
+{
+    const float f = __builtin_amdgcn_sqrtf(r2) * ph.spm;
+    *sure = f >= ph.far_min || __builtin_fabsf(__builtin_amdgcn_fractf(f) - 0.5f) < ph.clear_half;
+    return f;
+}
+
+/* Shell of a deposit at squared radius r2 (photon.c:26-29): with
+ * f = RN(RN(sqrt(r2)) * spm), whether f < shell_max (else the overflow
+ * shell) and floor(f).  The hardware square root is within one ulp of the
+ * correctly rounded one (the one-ulp correction of sqrt_cr_walk is exact on
+ * every float), so f' = RN(v_sqrt(r2) * spm) is within 3 ulp of f and has
+ * the same floor and the same side of shell_max when it is clear of every
+ * integer by eps or lies at or beyond shell_max + 1 (checked on every float
+ * r2 by tiny_mc_gpu_check_shell_index).  Otherwise, for a lane near a shell
+ * boundary (a few per ten thousand at 101 shells) or a radius below about
+ * 2^-48, the whole wave takes the IEEE square root. */
+__device__ __forceinline__ uint32_t shell_index(float r2, const Physics& ph, bool* inner)
+{
+    bool sure;
+    float f = shell_coord_short(r2, ph, &sure);
+    if (__builtin_expect(__ballot(!sure) != 0, 0))
+        f = sure ? f : sqrt_rn(r2) * ph.spm;
+    *inner = f < ph.shell_max;
+    return (uint32_t)f;
+}
 
 __device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon, uint32_t key0, uint32_t key1)
 {
@@ -86,10 +131,10 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
 
     /* absorb (photon.c:26-32) */
     const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
-    const float f = sqrt_radius(r2) * ph.spm;
-    const bool inner = f < ph.shell_max; /* else the overflow shell, ph.last_shell */
+    bool inner; /* else the overflow shell, ph.last_shell */
+    const uint32_t shell = shell_index(r2, ph, &inner);
     if (deposit)
-        tally_add(lt, inner, (uint32_t)f, fx_heat(p.weight), fx_heat2(p.weight));
+        tally_add(lt, inner, shell, fx_heat(p.weight), fx_heat2(p.weight));
     p.weight = p.weight * ph.albedo;
 
     /* new direction (photon.c:34-43) */

@@ -192,7 +192,9 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
 template <uint32_t SAMPLER, uint32_t MATH>
 __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
 {
-    const Physics phys = { a.albedo, a.spm, a.shell_max, a.shells - 1, (uint32_t)a.seed, (uint32_t)(a.seed >> 32) };
+    Physics phys = { a.albedo, a.spm, a.shell_max, a.shells - 1, (uint32_t)a.seed, (uint32_t)(a.seed >> 32), 0.0f,
+                     0.0f };
+    set_shell_bounds(phys);
     walk_body<Photon, Physics, SAMPLER, MATH>(a, phys);
 }
 
@@ -307,6 +309,51 @@ __global__ void check_short_op_kernel(uint32_t first, uint64_t count, unsigned l
         atomicAdd(&out[0], bad);
         atomicMin(&out[1], lowest);
     }
+}
+
+/* Checks shell_index's claim on the squared radii r2 with bit patterns
+ * [first, first + count) for one set's spm and shell_max: where the short
+ * coordinate is marked sure, its floor and its side of shell_max must be
+ * those of RN(RN(sqrt(r2)) * spm).  out[0] += violations, out[1] = min of
+ * the violating patterns, out[2] += radii not marked sure. */
+__global__ void check_shell_kernel(float spm, float shell_max, uint32_t first, uint64_t count,
+                                   unsigned long long* out)
+{
+    Physics ph = { 0.0f, spm, shell_max, 0u, 0u, 0u, 0.0f, 0.0f };
+    set_shell_bounds(ph);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bad = 0, lowest = ~0ull, unsure = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint32_t bits = first + (uint32_t)i;
+        const float r2 = __uint_as_float(bits);
+        bool sure;
+        const float fs = shell_coord_short(r2, ph, &sure);
+        if (!sure) {
+            ++unsure;
+            continue;
+        }
+        const float fe = sqrt_rn(r2) * spm;
+        const bool in_s = fs < shell_max, in_e = fe < shell_max;
+        if (in_s != in_e || (in_e && (uint32_t)fs != (uint32_t)fe)) {
+            ++bad;
+            lowest = bits < lowest ? bits : lowest;
+        }
+    }
+    if (bad) {
+        atomicAdd(&out[0], bad);
+        atomicMin(&out[1], lowest);
+    }
+    if (unsure)
+        atomicAdd(&out[2], unsure);
+}
+
+hipError_t launch_check_shell(float spm, float shell_max, uint32_t first, uint64_t count, unsigned long long* out,
+                              hipStream_t stream)
+{
+    if (count == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(check_shell_kernel, dim3(8192), dim3(256), 0, stream, spm, shell_max, first, count, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_check_short_op(uint32_t op, uint32_t first, uint64_t count, unsigned long long* out,

@@ -26,10 +26,11 @@ from . import _lib as L
 
 STRESS = dict(mu_a=0.01, mu_s=100.0, shells=401, microns_per_shell=10.0)
 MODES = [
-    ("fp64", L.PRECISION_FP64),
-    ("fp32", L.PRECISION_FP32),
-    ("fp32_weight", L.PRECISION_FP32_WEIGHT),
-    ("fp32_compensated", L.PRECISION_FP32_COMPENSATED),
+    ("fp64", L.PRECISION_FP64, L.MATH_HARDWARE),
+    ("fp32", L.PRECISION_FP32, L.MATH_HARDWARE),
+    ("fp32_own_math", L.PRECISION_FP32, L.MATH_OWN),
+    ("fp32_weight", L.PRECISION_FP32_WEIGHT, L.MATH_HARDWARE),
+    ("fp32_compensated", L.PRECISION_FP32_COMPENSATED, L.MATH_HARDWARE),
 ]
 
 
@@ -48,7 +49,7 @@ def _to_words(vals):
     return out
 
 
-def run_mode(params, seed, precision, parts=1, progress=None):
+def run_mode(params, seed, precision, parts=1, progress=None, math=L.MATH_HARDWARE):
     """Walks params.photons photons in `parts` launches over consecutive
     photon ranges (their integer tallies add exactly)."""
     acc, inter, ms = None, 0, 0.0
@@ -58,7 +59,7 @@ def run_mode(params, seed, precision, parts=1, progress=None):
         if hi == lo:
             continue
         p = L.Params(params.mu_a, params.mu_s, params.shells, params.microns_per_shell, hi - lo)
-        tally, st = L.run_raw(p, seed, first_photon=lo, precision=precision)
+        tally, st = L.run_raw(p, seed, first_photon=lo, precision=precision, math=math)
         acc = _add_tallies(acc, tally)
         inter += st.interactions
         ms += st.kernel_ms
@@ -90,11 +91,12 @@ def study(photons, seed=1, parts=1, modes=MODES, progress=None):
     params = L.Params(photons=photons, **STRESS)
     out = {"physics": dict(STRESS), "photons": photons, "seed": seed, "variants": {}}
     ref = None
-    for name, prec in modes:
+    for name, prec, math in modes:
         t0 = time.perf_counter()
-        heat, heat2, inter, ms = run_mode(params, seed, prec, parts, progress)
+        heat, heat2, inter, ms = run_mode(params, seed, prec, parts, progress, math)
         rec = {
             "precision": prec,
+            "math": math,
             "kernel_s": ms / 1e3,
             "wall_s": time.perf_counter() - t0,
             "photons_per_s": photons / (ms / 1e3) if ms > 0 else None,

@@ -67,9 +67,10 @@ def test_device_philox_kat(tmc, orc):
 
 
 def test_block0_shortcut_equals_full_rounds(tmp_path):
-    """tmc_philox4x32_10_block0 (round 1's photon part precomputed) equals the
-    full ten rounds of block (k, 0) on random counters and keys (host build
-    of the same header the kernels include)."""
+    """tmc_philox4x32_10_block0 (round 1's photon part precomputed) and its
+    form with the round keys precomputed (tmc_philox4x32_10_block0_ks) equal
+    the full ten rounds of block (k, 0) on random counters and keys (host
+    build of the same header the kernels include)."""
     src = tmp_path / "b0.c"
     src.write_text(r'''
 #include <stdio.h>
@@ -83,7 +84,10 @@ int main(void) {
         if (i & 1) k &= 0xFFFFu;
         tmc_u32x4 a = tmc_philox4x32_10(k, 0u, c2, c3, k0, k1);
         tmc_u32x4 b = tmc_philox4x32_10_block0(k, tmc_philox_photon_init(c2, c3, k0, k1), k0, k1);
+        tmc_u32x4 c = tmc_philox4x32_10_block0_ks(k, tmc_philox_photon_init(c2, c3, k0, k1),
+                                                  tmc_philox_key_schedule(k0, k1));
         bad += a.x0 != b.x0 || a.x1 != b.x1 || a.x2 != b.x2 || a.x3 != b.x3;
+        bad += a.x0 != c.x0 || a.x1 != c.x1 || a.x2 != c.x2 || a.x3 != c.x3;
     }
     printf("%u\n", bad);
     return 0;

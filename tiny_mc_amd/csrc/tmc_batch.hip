@@ -130,7 +130,17 @@ __device__ __forceinline__ unsigned long long take_sys(unsigned long long* p)
  * memory written before the launch; serving tables are written by the host
  * while the kernel runs, so lane 0 reads them with system-scope loads and
  * broadcasts the words. */
-template <bool SERVE>
+/* Whether walk_multi keeps a set's Philox round keys in scalar registers:
+ * the batch kernel with the analytic sampler has the registers to spare;
+ * the serving kernel and the rejection sampler (more live state) would spill
+ * them, so there the step forms the keys from the seed. */
+template <uint32_t SAMPLER, bool SERVE>
+__host__ __device__ constexpr bool batch_keys()
+{
+    return !SERVE && SAMPLER == 0u;
+}
+
+template <uint32_t SAMPLER, bool SERVE>
 __device__ __forceinline__ void load_set(SetState& st, const BatchArgs& a, uint32_t index, uint32_t lane)
 {
     /* words: seed, first_photon, tally_offset, (albedo | spm << 32),
@@ -161,7 +171,7 @@ __device__ __forceinline__ void load_set(SetState& st, const BatchArgs& a, uint3
     st.phys.shell_max = __uint_as_float((uint32_t)w[4]);
     st.shells = (uint32_t)(w[4] >> 32);
     st.phys.last_shell = st.shells - 1;
-    set_shell_bounds(st.phys);
+    set_derived<batch_keys<SAMPLER, SERVE>()>(st.phys);
     st.kcap = (uint32_t)w[5];
     st.done = a.done + 2 * (uint64_t)index;
     st.photons = w[6];
@@ -321,6 +331,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     bool have_pending = false, queue_done = false;
     BatchChunk pend = { 0u, 0u, 0ull };
     uint64_t wave_interactions = 0, wave_truncated = 0, completed = 0, absorbed = 0;
+    uint64_t stepped = 0; /* lanes that walked in the previous iteration */
     uint64_t wave_idle = 0; /* batch: lane-steps spent without a photon while the wave steps */
     uint32_t since_flush = 0;
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
@@ -334,6 +345,9 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
 
     for (;;) {
         uint64_t dead = __ballot(ph.weight < 0);
+        /* lanes whose photon ended in the previous step (roulette or cap) */
+        completed += (uint64_t)__popcll(dead & stepped);
+        stepped = 0;
         if (dead != 0) {
             if (SERVE && poll_wait && dead != ~0ull) {
                 --poll_wait; /* live lanes keep walking; poll again later */
@@ -370,7 +384,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                     flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
                 since_flush = 0;
                 cur_set = pend.set;
-                load_set<SERVE>(st, a, cur_set & slot_mask, lane);
+                load_set<SAMPLER, SERVE>(st, a, cur_set & slot_mask, lane);
                 pool_next = pend.first;
                 pool_end = pend.first + pend.count;
                 have_pending = false;
@@ -398,9 +412,8 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
              * costs no extra vector cycles, and with no lane-masked region
              * around the photon state the compiler keeps that state in place
              * (a masked step made it copy the state in and out each step) */
-            const bool was_live = ph.weight >= 0;
-            const bool survives = step<SAMPLER, MATH>(ph, st.phys, lt, was_live);
-            ph.weight = was_live && survives ? ph.weight : -1.0f; /* stays -1 at albedo 0 */
+            if (ph.weight >= 0 && !step<SAMPLER, MATH, batch_keys<SAMPLER, SERVE>()>(ph, st.phys, lt))
+                ph.weight = -1;
             /* photons that reached the absorption cap stop (rare) */
             const uint64_t capped = __ballot(ph.k >= st.kcap) & __ballot(ph.weight >= 0);
             if (capped != 0) {
@@ -408,12 +421,15 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 if (ph.k >= st.kcap)
                     ph.weight = -1;
             }
-            completed += (uint64_t)__popcll(live & __ballot(ph.weight < 0));
+            stepped = live;
 
-            /* wave-uniform; readfirstlane keeps it in a scalar register */
+            /* wave-uniform: the empty asm pins it to a scalar register (the
+             * compiler otherwise carries it in a vector one and reads it
+             * back every step) */
             since_flush = __builtin_amdgcn_readfirstlane(since_flush + 1u);
+            asm volatile("" : "+s"(since_flush));
             if (since_flush >= a.flush_interval) {
-                /* overflow guard only: completions are published on switch / idle */
+                /* overflow guard: completions are published on switch / idle */
                 since_flush = 0;
                 flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
             }

@@ -110,6 +110,45 @@ TMC_HD tmc_u32x4 tmc_philox4x32_10_block0(uint32_t k, tmc_philox_photon pp, uint
     return r;
 }
 
+/* Round keys of block (k, 0) for rounds 1-9: key0 + r W0 and key1 + r W1.
+ * They depend only on the seed, so a walk computes them once per parameter
+ * set and keeps them in scalar registers. */
+typedef struct tmc_philox_keys {
+    uint32_t k0[10], k1[10];
+} tmc_philox_keys;
+
+TMC_HD tmc_philox_keys tmc_philox_key_schedule(uint32_t k0, uint32_t k1)
+{
+    tmc_philox_keys ks;
+    for (int r = 0; r < 10; ++r) {
+        ks.k0[r] = k0 + (uint32_t)r * TMC_PHILOX_W0;
+        ks.k1[r] = k1 + (uint32_t)r * TMC_PHILOX_W1;
+    }
+    return ks;
+}
+
+/* tmc_philox4x32_10_block0 with the round keys precomputed. */
+TMC_HD tmc_u32x4 tmc_philox4x32_10_block0_ks(uint32_t k, tmc_philox_photon pp, tmc_philox_keys ks)
+{
+    const uint64_t q0 = (uint64_t)k * TMC_PHILOX_M0;
+    uint32_t c0 = pp.c0r1, c1 = pp.c1r1, c2 = (uint32_t)(q0 >> 32) ^ pp.c3k1, c3 = (uint32_t)q0;
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int round = 1; round < 10; ++round) {
+        const uint64_t p0 = (uint64_t)c0 * TMC_PHILOX_M0;
+        const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
+        const uint32_t n0 = tmc_xor3((uint32_t)(p1 >> 32), c1, ks.k0[round]);
+        const uint32_t n2 = tmc_xor3((uint32_t)(p0 >> 32), c3, ks.k1[round]);
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    tmc_u32x4 r = { c0, c1, c2, c3 };
+    return r;
+}
+
 /* 32 random bits -> uniform in (0, 1]: ((r >> 8) + 1) * 2^-24, exact in
  * binary32.  The reference maps rand() to [0, 1] (photon.c:21); excluding 0
  * keeps -log(xi) finite, which the reference does not guarantee. */

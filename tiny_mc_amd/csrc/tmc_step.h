@@ -27,17 +27,33 @@ struct Physics {
     float shell_max; /* (float)(shells - 1) */
     uint32_t last_shell; /* shells - 1 */
     uint32_t key0, key1; /* Philox key = 64-bit seed */
-    float far_min; /* shell_max + 1, set by set_shell_bounds */
-    float clear_half; /* 0.5 - eps, set by set_shell_bounds */
+    float far_min; /* shell_max + 1, set by set_derived */
+    float clear_half; /* 0.5 - eps, set by set_derived */
+    tmc_philox_keys keys; /* round keys of (key0, key1), set by set_derived */
 };
 
-/* Bounds of shell_index for the set's shell_max: a shell coordinate computed
+/* Per-set constants derived on the device: the Philox round keys, and the
+ * bounds of shell_index for the set's shell_max: a shell coordinate computed
  * from the hardware square root is within 3 ulp of the exact one (below), so
  * it has the exact floor if it lies more than eps = 8 ulp from every integer,
  * with the ulp of the floats just below 2^e > shell_max + 2; at or beyond
  * shell_max + 1 it is in the overflow shell either way. */
-__device__ __forceinline__ void set_shell_bounds(Physics& ph)
+template <bool PIN_KEYS = false>
+__device__ __forceinline__ void set_derived(Physics& ph)
 {
+    ph.keys = tmc_philox_key_schedule(ph.key0, ph.key1);
+    if (PIN_KEYS) {
+        /* where the keys change inside the walk loop (a new parameter set),
+         * the compiler recomputes each round key inside the step (18 scalar
+         * additions per absorption); the empty asm keeps them in scalar
+         * registers instead, for kernels with the registers to spare */
+#pragma unroll
+        for (int r = 1; r < 10; ++r) {
+            ph.keys.k0[r] = __builtin_amdgcn_readfirstlane(ph.keys.k0[r]);
+            ph.keys.k1[r] = __builtin_amdgcn_readfirstlane(ph.keys.k1[r]);
+            asm volatile("" : "+s"(ph.keys.k0[r]), "+s"(ph.keys.k1[r]));
+        }
+    }
     const int e = __builtin_amdgcn_frexp_expf(ph.shell_max + 2.0f); /* shell_max + 2 < 2^e */
     /* wave-uniform: keep the bounds in scalar registers */
     ph.far_min = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(ph.shell_max + 1.0f)));
@@ -117,11 +133,14 @@ __device__ __forceinline__ void tally_add(const LaneTally& lt, bool inner, uint3
  * roulette kills the photon.
  * MATH (TMC_MATH_*) selects how -ln u, sin/cos(2 pi u') and the analytic
  * sampler's sin(theta) are evaluated; everything else is the same
- * arithmetic in both modes. */
-template <uint32_t SAMPLER, uint32_t MATH>
+ * arithmetic in both modes.  STORED_KEYS takes the Philox round keys from
+ * ph.keys; otherwise they are formed from the seed inside the step (fewer
+ * live registers, for kernels short of scalar registers). */
+template <uint32_t SAMPLER, uint32_t MATH, bool STORED_KEYS = true>
 __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt, bool deposit = true)
 {
-    const tmc_u32x4 r = tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1); /* block (k, 0) */
+    const tmc_u32x4 r = STORED_KEYS ? tmc_philox4x32_10_block0_ks(p.k, p.pp, ph.keys) /* block (k, 0) */
+                                    : tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1);
 
     /* move (photon.c:21-24) */
     const float t = step_length<MATH>(r.x0);

@@ -192,9 +192,14 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
 template <uint32_t SAMPLER, uint32_t MATH>
 __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
 {
-    Physics phys = { a.albedo, a.spm, a.shell_max, a.shells - 1, (uint32_t)a.seed, (uint32_t)(a.seed >> 32), 0.0f,
-                     0.0f };
-    set_shell_bounds(phys);
+    Physics phys = {};
+    phys.albedo = a.albedo;
+    phys.spm = a.spm;
+    phys.shell_max = a.shell_max;
+    phys.last_shell = a.shells - 1;
+    phys.key0 = (uint32_t)a.seed;
+    phys.key1 = (uint32_t)(a.seed >> 32);
+    set_derived(phys);
     walk_body<Photon, Physics, SAMPLER, MATH>(a, phys);
 }
 
@@ -319,8 +324,10 @@ __global__ void check_short_op_kernel(uint32_t first, uint64_t count, unsigned l
 __global__ void check_shell_kernel(float spm, float shell_max, uint32_t first, uint64_t count,
                                    unsigned long long* out)
 {
-    Physics ph = { 0.0f, spm, shell_max, 0u, 0u, 0u, 0.0f, 0.0f };
-    set_shell_bounds(ph);
+    Physics ph = {};
+    ph.spm = spm;
+    ph.shell_max = shell_max;
+    set_derived(ph);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     unsigned long long bad = 0, lowest = ~0ull, unsure = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {

@@ -279,17 +279,27 @@ __device__ __forceinline__ void finish_request(const BatchArgs& a, const SetStat
 }
 
 /* Flushes the slot into the set's sums, then publishes the photons completed
- * since the last publication (and their absorptions).  Serving: the sums and
+ * since the last publication (and their absorptions) to the set, and the
+ * absorptions and idle lane-steps (batch) of the steps since then to the
+ * launch counters (published here rather than summed per step, to keep the
+ * walk loop's scalar registers few).  Serving: the sums and
  * the counters are system-scope atomics, and the wave waits for every sum
  * update to complete before it adds to the counters, so the wave whose
  * count completes the request finds final sums. */
 template <bool SERVE>
 __device__ __forceinline__ void flush_and_publish(unsigned long long* slot, const BatchArgs& a, const SetState& st,
-                                                  uint32_t lane, uint64_t& completed, uint64_t& absorbed)
+                                                  uint32_t lane, uint64_t& completed, uint64_t& absorbed,
+                                                  uint32_t& steps)
 {
     flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
     if (!completed && !absorbed)
         return;
+    if (lane == 0) {
+        atomicAdd(a.counters + 1, (unsigned long long)absorbed);
+        if (!SERVE)
+            atomicAdd(a.counters + 4, (unsigned long long)steps * 64u - absorbed);
+    }
+    steps = 0;
     if (SERVE) {
         wait_vmem();
         uint32_t last = 0;
@@ -330,9 +340,9 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     uint64_t pool_next = 0, pool_end = 0; /* photon offsets within cur_set */
     bool have_pending = false, queue_done = false;
     BatchChunk pend = { 0u, 0u, 0ull };
-    uint64_t wave_interactions = 0, wave_truncated = 0, completed = 0, absorbed = 0;
+    uint64_t completed = 0, absorbed = 0;
+    uint32_t steps = 0; /* steps since the last publication */
     uint64_t stepped = 0; /* lanes that walked in the previous iteration */
-    uint64_t wave_idle = 0; /* batch: lane-steps spent without a photon while the wave steps */
     uint32_t since_flush = 0;
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
     uint32_t poll_wait = 0; /* serving: steps before re-polling an unpublished ticket */
@@ -374,14 +384,14 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                      * active, when every outstanding ticket lies past the
                      * tail. */
                     if (cur_set != 0xFFFFFFFFu)
-                        flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
+                        flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
                     __builtin_amdgcn_s_sleep(64);
                 }
             }
             if (have_pending && dead == ~0ull) {
                 /* drained: hand the finished set's sums over, switch sets */
                 if (cur_set != 0xFFFFFFFFu)
-                    flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
+                    flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
                 since_flush = 0;
                 cur_set = pend.set;
                 load_set<SAMPLER, SERVE>(st, a, cur_set & slot_mask, lane);
@@ -402,22 +412,19 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
         if (live == 0 && queue_done && !have_pending)
             break;
         if (live != 0) {
-            const uint64_t n_live = (uint64_t)__popcll(live);
-            wave_interactions += n_live;
-            absorbed += n_live;
-            if (!SERVE)
-                wave_idle += 64u - n_live;
+            absorbed += (uint64_t)__popcll(live);
+            ++steps;
 
-            /* every lane steps, only live ones deposit: a lane without a photon
-             * costs no extra vector cycles, and with no lane-masked region
-             * around the photon state the compiler keeps that state in place
-             * (a masked step made it copy the state in and out each step) */
+            /* the loop has one back-edge (no continue): with several, the
+             * compiler copied the photon state in and out of the lane-masked
+             * step every iteration */
             if (ph.weight >= 0 && !step<SAMPLER, MATH, batch_keys<SAMPLER, SERVE>()>(ph, st.phys, lt))
                 ph.weight = -1;
             /* photons that reached the absorption cap stop (rare) */
             const uint64_t capped = __ballot(ph.k >= st.kcap) & __ballot(ph.weight >= 0);
             if (capped != 0) {
-                wave_truncated += (uint64_t)__popcll(capped);
+                if (lane == 0)
+                    atomicAdd(a.counters + 2, (unsigned long long)__popcll(capped));
                 if (ph.k >= st.kcap)
                     ph.weight = -1;
             }
@@ -436,13 +443,8 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
         }
     }
     if (cur_set != 0xFFFFFFFFu)
-        flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed);
+        flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
     if (lane == 0) {
-        atomicAdd(a.counters + 1, (unsigned long long)wave_interactions);
-        if (wave_truncated)
-            atomicAdd(a.counters + 2, (unsigned long long)wave_truncated);
-        if (!SERVE)
-            atomicAdd(a.counters + 4, (unsigned long long)wave_idle);
         /* serving: the last wave to leave rewinds the ticket counter for the
          * next launch (no wave takes a ticket after it has left) */
         if (SERVE && atomicAdd(a.counters + 3, 1ull) == a.total_waves - 1) {

@@ -168,7 +168,7 @@ def sweep_leg():
         "speedup_vs_one_launch_per_set": per_set_s / batched_s,
         "chunks": st.chunks,
         "lane_utilisation": st.lane_utilisation,
-        "launch": {"grid": st.grid, "block": st.block, "waves_per_simd": st.waves_per_simd,
+        "launch": {"passes": st.passes, "grid": st.grid, "block": st.block, "waves_per_simd": st.waves_per_simd,
                    "copies": st.copies, "chunk_work": st.chunk_work, "hip_graph": True},
     }
 

@@ -187,7 +187,8 @@ int tiny_mc_gpu_check_shell_index(const tmc_params* p, uint32_t first_bits, uint
                                   uint32_t* first_violation, uint64_t* unsure);
 
 /* ------------------------------------------------------------------------
- * Batched sweep: many parameter sets in one persistent launch.
+ * Batched sweep: many parameter sets in one persistent launch (one per
+ * LDS size class of the sets, up to three, back to back).
  *
  * No counterpart in the reference (it has one compile-time parameter set);
  * this is the loop of tiny_mc.c:47-49 run for many params.h configurations
@@ -228,6 +229,9 @@ typedef struct tmc_batch_stats {
     uint64_t idle_lane_steps; /* lanes without a photon while their wave stepped
                                  (drains at set switches and the end of the queue):
                                  lane utilisation = interactions / (interactions + idle) */
+    uint32_t passes; /* launches, one per LDS size class of the sets (grid, block,
+                        waves_per_simd, chunk_work above: the pass with the most work) */
+    uint32_t reserved;
 } tmc_batch_stats;
 
 typedef struct tmc_batch_plan tmc_batch_plan;
@@ -238,9 +242,11 @@ void tmc_batch_options_default(tmc_batch_options* opt);
  * tally (layout as tiny_mc_gpu_run_ex) starts at word 4 * sum_{j<i} shells_j. */
 uint64_t tmc_batch_tally_words(const tmc_set* sets, uint32_t nsets);
 
-/* Validates the sets, builds the chunk queue (longest first) and uploads the
- * set table and queue to the device; with use_graph, captures the steady
- * launch sequence (clear, walk, copy back) as a HIP graph. */
+/* Validates the sets, builds the chunk queues (longest first) and uploads the
+ * set table and queues to the device; with use_graph, captures the steady
+ * launch sequence (clear, walks, copy back) as a HIP graph.  Sets whose
+ * shell counts allow more resident waves (smaller per-wave LDS slots) run in
+ * a launch of their own when they carry at least a tenth of the work. */
 int tmc_batch_plan_create(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* opt,
                           tmc_batch_plan** plan);
 

@@ -111,3 +111,20 @@ def test_max_shells(tmc):
     assert st.block == 64
     single, _ = tmc.run_raw(p, 1)
     np.testing.assert_array_equal(flat, single)
+
+
+def test_lds_class_passes(tmc):
+    """Sets whose shell counts allow different resident waves run as separate
+    launches (one per LDS size class carrying at least a tenth of the work),
+    and every set still equals the same set run alone."""
+    small = [(tmc.Params(shells=51, microns_per_shell=100.0, photons=40000), s) for s in range(4)]
+    big = [(tmc.Params(shells=401, microns_per_shell=12.5, photons=40000), 10 + s) for s in range(4)]
+    huge = [(tmc.Params(shells=2001, microns_per_shell=2.5, photons=60000), 20)]
+    for sets, passes in ((small + big, 2), (small + big[:1] + [(tmc.Params(shells=401, photons=500), 30)], 2),
+                         (small + [(tmc.Params(shells=401, photons=2000), 31)], 1), (small + big + huge, 3)):
+        flat, st = tmc.batch_run(sets)
+        assert st.passes == passes, (passes, st)
+        for i, (a, b) in enumerate(zip(tmc.split_tallies(sets, flat), alone(tmc, sets))):
+            np.testing.assert_array_equal(a, b, err_msg=f"set {i} of a {passes}-pass batch")
+        assert st.photons == sum(p.photons for p, _ in sets)
+        assert (st.interactions + st.idle_lane_steps) % 64 == 0

@@ -341,6 +341,8 @@ class _BatchStats(ctypes.Structure):
         ("copies", ctypes.c_uint32),
         ("waves_per_simd", ctypes.c_uint32),
         ("idle_lane_steps", ctypes.c_uint64),
+        ("passes", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -360,6 +362,7 @@ class BatchStats:
     copies: int
     waves_per_simd: int
     idle_lane_steps: int
+    passes: int = 1
 
     @property
     def lane_utilisation(self):
@@ -420,7 +423,7 @@ def _sets_array(sets):
 def _batch_stats(st):
     return BatchStats(st.sets, st.photons, st.interactions, st.truncated, st.chunks, st.chunk_work,
                       st.expected_interactions, st.kernel_ms, st.total_ms, st.grid, st.block, st.copies,
-                      st.waves_per_simd, st.idle_lane_steps)
+                      st.waves_per_simd, st.idle_lane_steps, st.passes)
 
 
 def split_tallies(sets, flat):

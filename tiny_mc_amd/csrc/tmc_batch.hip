@@ -12,7 +12,7 @@
  * Tallies: every wave owns an LDS slot of shells_max shells (1 or 2 copies,
  * lanes 0-31 / 32-63 in separate copies to halve same-address collisions),
  * flushed into the set's 128-bit global sums when the wave changes set,
- * every flush_interval steps, and at exit.  After each flush the wave adds
+ * every TMC_SLOT_FLUSH_STEPS steps, and at exit.  After each flush the wave adds
  * the photons it completed for that set to the set's done counter (release
  * ordering), so done == photons means the set's tallies are complete.
  *
@@ -198,7 +198,7 @@ __device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t la
     if (!SERVE) {
         unsigned long long c = 0;
         if (lane == 0)
-            c = atomicAdd(a.counters, 1ull);
+            c = atomicAdd(a.counters + a.cursor, 1ull);
         c = rfl64(c);
         if (c >= a.nchunks)
             return CLAIM_DONE;
@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
              * back every step) */
             since_flush = __builtin_amdgcn_readfirstlane(since_flush + 1u);
             asm volatile("" : "+s"(since_flush));
-            if (since_flush >= a.flush_interval) {
+            if (since_flush >= TMC_SLOT_FLUSH_STEPS) {
                 /* overflow guard: completions are published on switch / idle */
                 since_flush = 0;
                 flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);

@@ -12,6 +12,13 @@
  * lanes run out of the set's photons) once per ~P photons per lane.  Chunks
  * are sorted by expected duration, longest first, which keeps the long
  * high-albedo chunks off the end of the launch.
+ *
+ * Passes: a wave's LDS slot holds the largest shell count of its launch, so
+ * a few sets with many shells would lower every wave's occupancy.  Sets are
+ * grouped by the launch shape their shell count allows (classes with under a
+ * tenth of the work join the next larger class), and each group runs as a
+ * launch of its own, one after the other on the same stream, with its own
+ * chunk queue and cursor; the tallies and counters are shared.
  */
 #include <hip/hip_runtime.h>
 
@@ -28,14 +35,32 @@
 
 #define TMC_EXPORT extern "C" __attribute__((visibility("default")))
 
+static_assert(TMC_BATCH_CURSOR0 + TMC_BATCH_MAX_PASSES <= TMC_BATCH_COUNTERS, "pass cursors beyond the counters");
+
+namespace {
+
+/* One launch of a batch: the sets of one LDS size class. */
+struct PassPlan {
+    uint32_t grid = 0, block = 0, wps = 0, shells_max = 1;
+    uint64_t nchunks = 0;
+    uint64_t chunk_work = 0;
+    double work = 0.0;
+    tmc::BatchArgs args;
+};
+
+} // namespace
+
 struct tmc_batch_plan {
     uint32_t nsets = 0;
     uint64_t tally_words = 0;
     uint64_t nchunks = 0;
     uint64_t photons = 0;
     double expected = 0.0;
+    /* shape of the pass with the most work (reported in the statistics) */
     uint64_t chunk_work = 0;
     uint32_t sampler = 0, math = 0, grid = 0, block = 0, copies = 1, wps = 0, shells_max = 1;
+    uint32_t npasses = 0;
+    PassPlan pass[TMC_BATCH_MAX_PASSES];
     bool use_graph = false;
     tmc::BatchSet* d_sets = nullptr;
     tmc::BatchChunk* d_chunks = nullptr;
@@ -46,7 +71,6 @@ struct tmc_batch_plan {
     unsigned long long* h_counters = nullptr; /* pinned, TMC_BATCH_COUNTERS words */
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    tmc::BatchArgs args;
 };
 
 namespace {
@@ -82,8 +106,11 @@ hipError_t enqueue_run(tmc_batch_plan* p, hipStream_t stream)
     hipError_t e = tmc::launch_batch_clear(p->d_tally, p->tally_words, p->d_done, p->nsets, p->d_counters, stream);
     if (e != hipSuccess)
         return e;
-    if (p->nchunks) {
-        e = tmc::launch_batch(p->sampler, p->math, p->args, p->grid, p->block, stream);
+    for (uint32_t j = 0; j < p->npasses; ++j) {
+        const PassPlan& q = p->pass[j];
+        if (!q.nchunks)
+            continue;
+        e = tmc::launch_batch(p->sampler, p->math, q.args, q.grid, q.block, stream);
         if (e != hipSuccess)
             return e;
     }
@@ -92,6 +119,41 @@ hipError_t enqueue_run(tmc_batch_plan* p, hipStream_t stream)
         return e;
     return hipMemcpyAsync(p->h_counters, p->d_counters, TMC_BATCH_COUNTERS * sizeof(uint64_t), hipMemcpyDeviceToHost,
                           stream);
+}
+
+/* Launch shape for per-wave LDS slots of shells_max shells: the requested
+ * block, or the largest of 256/128/64 whose slots fit on a CU, with the
+ * resident workgroups capped by waves_per_simd when requested.  *fits is
+ * false when no block fits. */
+hipError_t launch_shape(const tmc_batch_plan* p, const tmc_batch_options* o, uint32_t cus, uint32_t shells_max,
+                        PassPlan* q, bool* fits)
+{
+    *fits = false;
+    std::vector<uint32_t> cand;
+    if (o->block)
+        cand.push_back(o->block);
+    else
+        cand = { 256u, 128u, 64u };
+    for (uint32_t b : cand) {
+        int n = 0;
+        if (tmc::batch_lds_bytes(b, shells_max, p->copies) <= 160u * 1024u) {
+            const hipError_t e = tmc::batch_occupancy(p->sampler, p->math, b, shells_max, p->copies, &n);
+            if (e != hipSuccess)
+                return e;
+        }
+        if (n <= 0)
+            continue;
+        const uint32_t waves_per_wg = b / 64u;
+        if (o->waves_per_simd)
+            n = std::min<int>(n, (int)std::max(1u, o->waves_per_simd * 4u / waves_per_wg));
+        q->block = b;
+        q->grid = (uint32_t)n * cus;
+        q->wps = (uint32_t)n * waves_per_wg / 4u;
+        q->shells_max = shells_max;
+        *fits = true;
+        return hipSuccess;
+    }
+    return hipSuccess;
 }
 
 int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, const tmc_batch_options* o,
@@ -147,56 +209,80 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
     p->shells_max = shells_max;
     p->expected = work;
 
-    /* launch shape: the requested block, or the largest of 256/128/64 whose
-     * per-wave LDS slots fit on a CU */
-    int wg_per_cu = 0;
-    uint32_t block = 0;
-    std::vector<uint32_t> cand;
-    if (o->block)
-        cand.push_back(o->block);
-    else
-        cand = { 256u, 128u, 64u };
-    for (uint32_t b : cand) {
-        int n = 0;
-        if (tmc::batch_lds_bytes(b, shells_max, copies) <= 160u * 1024u
-            && tmc::batch_occupancy(p->sampler, p->math, b, shells_max, copies, &n) != hipSuccess) {
+    /* LDS size classes: the distinct shell counts, ascending, grouped by the
+     * launch shape they allow */
+    std::vector<uint32_t> svals;
+    for (uint32_t i = 0; i < nsets; ++i)
+        svals.push_back(sets[i].params.shells);
+    std::sort(svals.begin(), svals.end());
+    svals.erase(std::unique(svals.begin(), svals.end()), svals.end());
+    std::vector<double> swork(svals.size(), 0.0);
+    for (uint32_t i = 0; i < nsets; ++i) {
+        const size_t k = std::lower_bound(svals.begin(), svals.end(), sets[i].params.shells) - svals.begin();
+        swork[k] += L[i] * (double)sets[i].params.photons;
+    }
+    struct Group {
+        uint32_t lo, hi;
+        PassPlan shape;
+    };
+    std::vector<Group> groups;
+    for (size_t k = 0; k < svals.size(); ++k) {
+        PassPlan s;
+        bool fits = false;
+        if (launch_shape(p, o, (uint32_t)c->compute_units, svals[k], &s, &fits) != hipSuccess) {
             plan_free(p);
             return TMC_E_HIP;
         }
-        if (n > 0) {
-            wg_per_cu = n;
-            block = b;
-            break;
+        if (!fits) {
+            plan_free(p);
+            return TMC_E_INVALID; /* LDS slots of this many shells do not fit */
+        }
+        s.work = swork[k];
+        if (!groups.empty() && groups.back().shape.grid == s.grid && groups.back().shape.block == s.block) {
+            groups.back().hi = svals[k];
+            groups.back().shape.shells_max = svals[k];
+            groups.back().shape.work += s.work;
+        } else {
+            groups.push_back({ svals[k], svals[k], s });
         }
     }
-    if (wg_per_cu <= 0) {
-        plan_free(p);
-        return TMC_E_INVALID; /* LDS slots of shells_max shells do not fit */
+    /* a class with under a tenth of the work joins its neighbour with more
+     * shells (whose slots hold both), and there are at most
+     * TMC_BATCH_MAX_PASSES passes */
+    while (groups.size() > 1) {
+        size_t m = 0;
+        for (size_t k = 1; k < groups.size(); ++k)
+            if (groups[k].shape.work < groups[m].shape.work)
+                m = k;
+        if (groups[m].shape.work >= 0.1 * work && groups.size() <= TMC_BATCH_MAX_PASSES)
+            break;
+        const size_t lo_i = m + 1 < groups.size() ? m : m - 1, hi_i = lo_i + 1;
+        Group g = groups[hi_i];
+        g.lo = groups[lo_i].lo;
+        g.shape.work += groups[lo_i].shape.work;
+        groups[lo_i] = g;
+        groups.erase(groups.begin() + (ptrdiff_t)hi_i);
     }
-    p->block = block;
-    const uint32_t waves_per_wg = block / 64u;
-    if (o->waves_per_simd) {
-        const uint32_t want = std::max(1u, o->waves_per_simd * 4u / waves_per_wg);
-        wg_per_cu = std::min<int>(wg_per_cu, (int)want);
-    }
-    p->grid = (uint32_t)wg_per_cu * (uint32_t)c->compute_units;
-    p->wps = (uint32_t)wg_per_cu * waves_per_wg / 4u;
-    const double total_waves = (double)p->grid * waves_per_wg;
 
-    /* chunk queue */
-    uint64_t cw = o->chunk_work;
-    if (!cw)
-        cw = (uint64_t)std::max(65536.0, work / (16.0 * total_waves));
-    p->chunk_work = cw;
+    /* chunk queues, one per pass */
     struct Item {
         tmc::BatchChunk ch;
         double dur;
     };
-    std::vector<Item> items;
+    std::vector<std::vector<Item>> items(groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) {
+        PassPlan& q = groups[g].shape;
+        const double waves = (double)q.grid * (q.block / 64u);
+        q.chunk_work = o->chunk_work ? o->chunk_work : (uint64_t)std::max(65536.0, q.work / (16.0 * waves));
+    }
     for (uint32_t i = 0; i < nsets; ++i) {
         const uint64_t N = sets[i].params.photons;
         if (!N)
             continue;
+        size_t g = 0;
+        while (sets[i].params.shells > groups[g].hi)
+            ++g;
+        const uint64_t cw = groups[g].shape.chunk_work;
         double per_lane = std::ceil((double)cw / (64.0 * L[i]));
         per_lane = std::min(std::max(per_lane, 1.0), (double)(1u << 24));
         uint64_t C = 64u * (uint64_t)per_lane;
@@ -208,14 +294,48 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
             C = 1ull << 31;
         for (uint64_t first = 0; first < N; first += C) {
             const uint64_t cnt = std::min(C, N - first);
-            items.push_back({ { i, (uint32_t)cnt, first }, (double)((cnt + 63) / 64) * L[i] });
+            items[g].push_back({ { i, (uint32_t)cnt, first }, (double)((cnt + 63) / 64) * L[i] });
         }
     }
-    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.dur > y.dur; });
-    p->nchunks = items.size();
-    std::vector<tmc::BatchChunk> hc(items.size());
-    for (size_t i = 0; i < items.size(); ++i)
-        hc[i] = items[i].ch;
+    for (auto& v : items)
+        std::stable_sort(v.begin(), v.end(), [](const Item& x, const Item& y) { return x.dur > y.dur; });
+    /* the pass with the longest chunk first, so that the launch sequence ends
+     * on short chunks */
+    std::vector<size_t> order(groups.size());
+    for (size_t g = 0; g < order.size(); ++g)
+        order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        const double dx = items[x].empty() ? 0.0 : items[x][0].dur, dy = items[y].empty() ? 0.0 : items[y][0].dur;
+        return dx > dy;
+    });
+    std::vector<tmc::BatchChunk> hc;
+    std::vector<uint64_t> first_chunk(groups.size());
+    p->npasses = (uint32_t)groups.size();
+    for (size_t j = 0; j < order.size(); ++j) {
+        const size_t g = order[j];
+        first_chunk[j] = hc.size();
+        for (const Item& it : items[g])
+            hc.push_back(it.ch);
+        p->pass[j] = groups[g].shape;
+        p->pass[j].nchunks = items[g].size();
+    }
+    p->nchunks = hc.size();
+    p->shells_max = shells_max;
+    for (uint32_t j = 0; j < p->npasses; ++j)
+        if (p->pass[j].nchunks > UINT32_MAX) {
+            plan_free(p);
+            return TMC_E_RANGE; /* a pass's queue length is a 32-bit kernel argument */
+        }
+    uint32_t main_pass = 0;
+    for (uint32_t j = 1; j < p->npasses; ++j)
+        if (p->pass[j].work > p->pass[main_pass].work)
+            main_pass = j;
+    if (p->npasses) {
+        p->grid = p->pass[main_pass].grid;
+        p->block = p->pass[main_pass].block;
+        p->wps = p->pass[main_pass].wps;
+        p->chunk_work = p->pass[main_pass].chunk_work;
+    }
 
     /* device memory */
     if (hipMalloc((void**)&p->d_sets, std::max<size_t>(1, nsets) * sizeof(tmc::BatchSet)) != hipSuccess
@@ -237,18 +357,19 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
         return TMC_E_HIP;
     }
 
-    std::memset(&p->args, 0, sizeof p->args);
-    p->args.sets = p->d_sets;
-    p->args.chunks = p->d_chunks;
-    p->args.nchunks = p->nchunks;
-    p->args.shells_max = shells_max;
-    p->args.copies = copies;
-    /* a wave alone adds to its slot: 64 lanes x flush_interval deposits of
-     * at most 2^43 (heat2 image of w = 1) per entry stay below 2^63 */
-    p->args.flush_interval = 1u << 14;
-    p->args.counters = p->d_counters;
-    p->args.tally = p->d_tally;
-    p->args.done = p->d_done;
+    for (uint32_t j = 0; j < p->npasses; ++j) {
+        tmc::BatchArgs& a = p->pass[j].args;
+        std::memset(&a, 0, sizeof a);
+        a.sets = p->d_sets;
+        a.chunks = p->d_chunks + first_chunk[j];
+        a.nchunks = (uint32_t)p->pass[j].nchunks;
+        a.shells_max = p->pass[j].shells_max;
+        a.copies = copies;
+        a.counters = p->d_counters;
+        a.cursor = TMC_BATCH_CURSOR0 + j;
+        a.tally = p->d_tally;
+        a.done = p->d_done;
+    }
 
     if (p->use_graph) {
         hipStream_t s = c->stream;
@@ -301,6 +422,8 @@ int plan_run_locked(tmc::Context* c, tmc_batch_plan* p, uint64_t* tallies, tmc_b
         st->block = p->block;
         st->copies = p->copies;
         st->waves_per_simd = p->wps;
+        st->passes = p->npasses;
+        st->reserved = 0;
     }
     return TMC_OK;
 }

@@ -53,14 +53,14 @@ struct ServeCache {
 struct BatchArgs {
     const BatchSet* sets; /* batch: device table; serve: host-coherent table */
     const BatchChunk* chunks; /* batch: device queue; serve: host-coherent ring */
-    uint64_t nchunks; /* batch: queue length */
+    uint32_t nchunks; /* batch: queue length of this pass */
+    uint32_t cursor; /* batch: index of this pass's chunk cursor in counters */
     uint32_t shells_max; /* LDS slot size in shells */
     uint32_t copies; /* tally copies per wave (1 or 2) */
-    uint32_t flush_interval; /* wave steps between slot flushes */
     uint32_t ring_mask; /* serve: ring size - 1 (power of two) */
-    unsigned long long* counters; /* [0] chunk cursor (serve: tickets of this launch) [1] interactions
-                                     [2] truncated [3] serve: waves that have left
-                                     [4] batch: idle lane-steps (TMC_BATCH_COUNTERS words) */
+    unsigned long long* counters; /* [0] serve: tickets of this launch [1] interactions [2] truncated
+                                     [3] serve: waves that have left [4] batch: idle lane-steps
+                                     [5 + j] batch: chunk cursor of pass j (TMC_BATCH_COUNTERS words) */
     unsigned long long* tally; /* tally pool */
     unsigned long long* done; /* per set: [2s] photons completed and flushed, [2s+1] absorptions */
     /* serving only */
@@ -75,8 +75,17 @@ struct BatchArgs {
     uint32_t refresh_ticks; /* realtime ticks (100 MHz) between reads of the host control words */
 };
 
+/* Wave steps between flushes of a wave's LDS slot: a wave alone adds to its
+ * slot, so 64 lanes x 2^14 deposits of at most 2^43 (the heat2 image of
+ * w = 1) per entry stay below 2^63.  A constant rather than a kernel
+ * argument: the walk loop keeps no register for it. */
+#define TMC_SLOT_FLUSH_STEPS (1u << 14)
+
 /* 64-bit counter words of a batched launch. */
 #define TMC_BATCH_COUNTERS 8
+/* Launches of a batch, each over the sets of one LDS-slot size class. */
+#define TMC_BATCH_MAX_PASSES 3
+#define TMC_BATCH_CURSOR0 5
 
 /* Serving: a set field of a chunk is (slot | generation << 16), so a slot
  * reused by a later request never matches a wave's stale set. */

@@ -503,7 +503,6 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
     s->args.nchunks = 0;
     s->args.shells_max = o.max_shells;
     s->args.copies = 1;
-    s->args.flush_interval = 1u << 14;
     s->args.ring_mask = (uint32_t)(o.ring_chunks - 1);
     s->args.counters = s->d_counters;
     s->args.tally = s->d_tally;

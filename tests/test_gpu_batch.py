@@ -103,6 +103,16 @@ def test_truncation_in_batch(tmc):
     assert st.truncated > 0
 
 
+def test_truncation_beyond_flush_interval_in_batch(tmc):
+    """A cap larger than a wave's flush interval (2^14 steps) in the batch."""
+    sets = [(tmc.Params(mu_a=0.01, mu_s=100.0, shells=401, microns_per_shell=10.0, photons=300), 3),
+            (tmc.Params(photons=20000), 4)]
+    flat, st = tmc.batch_run(sets, tmc.BatchOptions(max_interactions=20000))
+    for (p, seed), t in zip(sets, tmc.split_tallies(sets, flat)):
+        np.testing.assert_array_equal(t, tmc.run_raw(p, seed, max_interactions=20000)[0])
+    assert st.truncated == 300
+
+
 def test_max_shells(tmc):
     """LDS slot of 8192 shells (one wave per workgroup) and the single-run
     kernel at the same size."""

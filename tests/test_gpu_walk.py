@@ -70,6 +70,25 @@ def test_truncation_matches_mirror(tmc, orc, hw_tables):
             assert st.truncated == 30000 and n == 30000
 
 
+def test_truncation_beyond_flush_interval(tmc, orc, hw_tables):
+    """Caps larger than the steps between two tally flushes: the walk checks
+    the cap only from the flush before some photon can reach it, and the
+    truncated walk still equals the mirror's (at albedo 0.9999 every photon
+    reaches 20000 absorptions; at 69070 the roulette at absorption 69063
+    kills most of them first)."""
+    phys = dict(mu_a=0.01, mu_s=100.0, shells=401, microns_per_shell=10.0)
+    for kmax, photons in ((20000, 512), (69070, 256)):
+        p = _params(tmc, phys, photons)
+        tally, st = tmc.run_raw(p, 8, max_interactions=kmax)
+        want, n = _mirror(orc, phys, 8, photons, max_k=kmax, tables=hw_tables)
+        np.testing.assert_array_equal(tally, want)
+        assert st.interactions == n
+        if kmax == 20000:
+            assert st.truncated == photons and n == photons * kmax
+        else:
+            assert 0 < st.truncated < photons
+
+
 def test_run_twice_identical_and_resume_adds(tmc):
     p = _params(tmc, DEFAULT, 300000)
     a, _ = tmc.run_raw(p, 99)

@@ -338,7 +338,9 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     SetState st;
     uint32_t cur_set = 0xFFFFFFFFu;
     uint64_t pool_next = 0, pool_end = 0; /* photon offsets within cur_set */
-    bool have_pending = false, queue_done = false;
+    /* wave flags, one scalar word */
+    enum : uint32_t { PENDING = 1u, QUEUE_DONE = 2u, CAPPING = 4u /* per-step cap checks until the next flush */ };
+    uint32_t flags = 0;
     BatchChunk pend = { 0u, 0u, 0ull };
     uint64_t completed = 0, absorbed = 0;
     uint32_t steps = 0; /* steps since the last publication */
@@ -361,7 +363,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
         if (dead != 0) {
             if (SERVE && poll_wait && dead != ~0ull) {
                 --poll_wait; /* live lanes keep walking; poll again later */
-            } else if (pool_next == pool_end && !have_pending && !queue_done) {
+            } else if (pool_next == pool_end && !(flags & (PENDING | QUEUE_DONE))) {
                 BatchChunk ch;
                 const ClaimResult r = try_claim<SERVE>(a, lane, ticket, ch);
                 if (r == CLAIM_GOT) {
@@ -370,10 +372,10 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                         pool_end = ch.first + ch.count;
                     } else {
                         pend = ch;
-                        have_pending = true;
+                        flags |= PENDING;
                     }
                 } else if (r == CLAIM_DONE) {
-                    queue_done = true;
+                    flags |= QUEUE_DONE;
                 } else if (SERVE && dead != ~0ull) {
                     poll_wait = 16;
                 } else if (SERVE) {
@@ -388,16 +390,18 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                     __builtin_amdgcn_s_sleep(64);
                 }
             }
-            if (have_pending && dead == ~0ull) {
+            if ((flags & PENDING) && dead == ~0ull) {
                 /* drained: hand the finished set's sums over, switch sets */
                 if (cur_set != 0xFFFFFFFFu)
                     flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
                 since_flush = 0;
                 cur_set = pend.set;
                 load_set<SAMPLER, SERVE>(st, a, cur_set & slot_mask, lane);
+                /* every lane is empty here: only a small cap needs checks */
+                flags = st.kcap <= TMC_SLOT_FLUSH_STEPS ? (flags | CAPPING) : (flags & ~CAPPING);
                 pool_next = pend.first;
                 pool_end = pend.first + pend.count;
-                have_pending = false;
+                flags &= ~PENDING;
             }
             const uint64_t avail = pool_end - pool_next;
             if (avail) {
@@ -409,7 +413,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             }
         }
         const uint64_t live = __ballot(ph.weight >= 0);
-        if (live == 0 && queue_done && !have_pending)
+        if (live == 0 && (flags & (PENDING | QUEUE_DONE)) == QUEUE_DONE)
             break;
         if (live != 0) {
             absorbed += (uint64_t)__popcll(live);
@@ -420,13 +424,16 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
              * step every iteration */
             if (ph.weight >= 0 && !step<SAMPLER, MATH, batch_keys<SAMPLER, SERVE>()>(ph, st.phys, lt))
                 ph.weight = -1;
-            /* photons that reached the absorption cap stop (rare) */
-            const uint64_t capped = __ballot(ph.k >= st.kcap) & __ballot(ph.weight >= 0);
-            if (capped != 0) {
-                if (lane == 0)
-                    atomicAdd(a.counters + 2, (unsigned long long)__popcll(capped));
-                if (ph.k >= st.kcap)
-                    ph.weight = -1;
+            /* photons that reached the absorption cap stop; checked only while
+             * some photon of the wave can reach it before the next flush */
+            if (flags & CAPPING) {
+                const uint64_t capped = __ballot(ph.k >= st.kcap) & __ballot(ph.weight >= 0);
+                if (capped != 0) {
+                    if (lane == 0)
+                        atomicAdd(a.counters + 2, (unsigned long long)__popcll(capped));
+                    if (ph.k >= st.kcap)
+                        ph.weight = -1;
+                }
             }
             stepped = live;
 
@@ -439,6 +446,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 /* overflow guard: completions are published on switch / idle */
                 since_flush = 0;
                 flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
+                flags = cap_within(ph, st.kcap, TMC_SLOT_FLUSH_STEPS) ? (flags | CAPPING) : (flags & ~CAPPING);
             }
         }
     }

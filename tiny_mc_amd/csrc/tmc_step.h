@@ -123,6 +123,19 @@ __device__ __forceinline__ void tally_add(const LaneTally& lt, bool inner, uint3
     atomicAdd(p + 1, b);
 }
 
+/* Whether a live photon of the wave can reach the absorption cap kcap
+ * within the next `steps` steps: its k grows by one per step, and photons
+ * launched meanwhile start at k = 0.  Evaluated at each periodic flush, it
+ * lets the walk skip the per-step cap check until the next one; with the
+ * default cap (2^32 - 1) the check never runs. */
+template <class PhotonT>
+__device__ __forceinline__ bool cap_within(const PhotonT& ph, uint32_t kcap, uint32_t steps)
+{
+    if (kcap <= steps)
+        return true;
+    return __ballot(ph.weight >= 0 && ph.k >= kcap - steps) != 0;
+}
+
 /* The roulette kills a photon when u > 0.1f (photon.c:46).  For
  * u = ((r >> 8) + 1) 2^-24 that is (r >> 8) + 1 > 0.1f * 2^24 = 1677721.625,
  * i.e. r >= 1677721 * 2^8: one integer compare on the Philox word. */

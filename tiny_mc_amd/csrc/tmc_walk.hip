@@ -116,6 +116,7 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
     PhotonT ph;
     launch_photon(ph, 0, phys.key0, phys.key1);
     ph.weight = -1;
+    bool capping = cap_within(ph, a.kcap, a.flush_interval); /* per-step cap checks until the next flush */
 
     for (;;) {
         const uint64_t dead = __ballot(ph.weight < 0);
@@ -164,13 +165,15 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
 
         if (ph.weight >= 0 && !step_of<SAMPLER, VARIANT>(ph, phys, lt))
             ph.weight = -1;
-        /* photons that reached the absorption cap stop (rare: a uniform
-         * branch around the bookkeeping) */
-        const uint64_t capped = __ballot(ph.k >= a.kcap) & __ballot(ph.weight >= 0);
-        if (capped != 0) {
-            wave_truncated += (uint64_t)__popcll(capped);
-            if (ph.k >= a.kcap)
-                ph.weight = -1;
+        /* photons that reached the absorption cap stop; checked only while
+         * some photon of the wave can reach it before the next flush */
+        if (capping) {
+            const uint64_t capped = __ballot(ph.k >= a.kcap) & __ballot(ph.weight >= 0);
+            if (capped != 0) {
+                wave_truncated += (uint64_t)__popcll(capped);
+                if (ph.k >= a.kcap)
+                    ph.weight = -1;
+            }
         }
 
         /* wave-uniform; readfirstlane keeps it in a scalar register */
@@ -178,6 +181,7 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
         if (since_flush >= a.flush_interval) {
             since_flush = 0;
             flush_tallies(lds_tally, hot, a.tally, a.shells, lane);
+            capping = cap_within(ph, a.kcap, a.flush_interval);
         }
     }
     flush_tallies(lds_tally, hot, a.tally, a.shells, lane);

@@ -104,13 +104,24 @@ def test_truncation_in_batch(tmc):
 
 
 def test_truncation_beyond_flush_interval_in_batch(tmc):
-    """A cap larger than a wave's flush interval (2^14 steps) in the batch."""
+    """Caps below and above the batch kernel's cap-test horizon (2^20 steps,
+    the most steps between two slot flushes): equal to the single-run
+    kernel (whose own horizon is a few thousand steps).  At albedo 0.99999
+    about a tenth of the photons survive the first roulette (~6.9e5
+    absorptions) and pass a cap of 2^20 + 5000."""
     sets = [(tmc.Params(mu_a=0.01, mu_s=100.0, shells=401, microns_per_shell=10.0, photons=300), 3),
             (tmc.Params(photons=20000), 4)]
     flat, st = tmc.batch_run(sets, tmc.BatchOptions(max_interactions=20000))
     for (p, seed), t in zip(sets, tmc.split_tallies(sets, flat)):
         np.testing.assert_array_equal(t, tmc.run_raw(p, seed, max_interactions=20000)[0])
     assert st.truncated == 300
+    kmax = (1 << 20) + 5000
+    sets = [(tmc.Params(mu_a=0.001, mu_s=100.0, shells=101, microns_per_shell=100.0, photons=256), 7)]
+    flat, st = tmc.batch_run(sets, tmc.BatchOptions(max_interactions=kmax))
+    single, ss = tmc.run_raw(sets[0][0], 7, max_interactions=kmax)
+    np.testing.assert_array_equal(flat, single)
+    assert st.truncated == ss.truncated and 0 < st.truncated < 256
+    assert st.interactions == ss.interactions
 
 
 def test_max_shells(tmc):

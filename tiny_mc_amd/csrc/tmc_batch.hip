@@ -12,7 +12,7 @@
  * Tallies: every wave owns an LDS slot of shells_max shells (1 or 2 copies,
  * lanes 0-31 / 32-63 in separate copies to halve same-address collisions),
  * flushed into the set's 128-bit global sums when the wave changes set,
- * every TMC_SLOT_FLUSH_STEPS steps, and at exit.  After each flush the wave adds
+ * every TMC_SLOT_FLUSH_DEPOSITS absorptions, and at exit.  After each flush the wave adds
  * the photons it completed for that set to the set's done counter (release
  * ordering), so done == photons means the set's tallies are complete.
  *
@@ -345,7 +345,6 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     uint64_t completed = 0, absorbed = 0;
     uint32_t steps = 0; /* steps since the last publication */
     uint64_t stepped = 0; /* lanes that walked in the previous iteration */
-    uint32_t since_flush = 0;
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
     uint32_t poll_wait = 0; /* serving: steps before re-polling an unpublished ticket */
 
@@ -394,7 +393,6 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 /* drained: hand the finished set's sums over, switch sets */
                 if (cur_set != 0xFFFFFFFFu)
                     flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
-                since_flush = 0;
                 cur_set = pend.set;
                 load_set<SAMPLER, SERVE>(st, a, cur_set & slot_mask, lane);
                 /* every lane is empty here: only a small cap needs checks */
@@ -416,6 +414,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
         if (live == 0 && (flags & (PENDING | QUEUE_DONE)) == QUEUE_DONE)
             break;
         if (live != 0) {
+            const uint64_t absorbed_before = absorbed;
             absorbed += (uint64_t)__popcll(live);
             ++steps;
 
@@ -437,14 +436,13 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             }
             stepped = live;
 
-            /* wave-uniform: the empty asm pins it to a scalar register (the
-             * compiler otherwise carries it in a vector one and reads it
-             * back every step) */
-            since_flush = __builtin_amdgcn_readfirstlane(since_flush + 1u);
-            asm volatile("" : "+s"(since_flush));
-            if (since_flush >= TMC_SLOT_FLUSH_STEPS) {
-                /* overflow guard: completions are published on switch / idle */
-                since_flush = 0;
+            /* overflow guard: a flush whenever the absorptions since the last
+             * publication (which flushes too) cross a multiple of
+             * TMC_SLOT_FLUSH_DEPOSITS, so no slot word takes more deposits
+             * between two flushes; completions are published on switch /
+             * idle.  The cap condition is evaluated at least as often: the
+             * other publications happen only when every lane is empty. */
+            if (((absorbed_before ^ absorbed) / TMC_SLOT_FLUSH_DEPOSITS) != 0) {
                 flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
                 flags = cap_within(ph, st.kcap, TMC_SLOT_FLUSH_STEPS) ? (flags | CAPPING) : (flags & ~CAPPING);
             }

@@ -75,11 +75,16 @@ struct BatchArgs {
     uint32_t refresh_ticks; /* realtime ticks (100 MHz) between reads of the host control words */
 };
 
-/* Wave steps between flushes of a wave's LDS slot: a wave alone adds to its
- * slot, so 64 lanes x 2^14 deposits of at most 2^43 (the heat2 image of
- * w = 1) per entry stay below 2^63.  A constant rather than a kernel
- * argument: the walk loop keeps no register for it. */
-#define TMC_SLOT_FLUSH_STEPS (1u << 14)
+/* Deposits between flushes of a wave's LDS slot: a wave alone adds to its
+ * slot, so 2^20 deposits of at most 2^43 (the heat2 image of w = 1) per
+ * entry stay below 2^63.  A power of two (the walk detects a crossing of its
+ * multiples in the running absorption count) and a constant rather than a
+ * kernel argument: the walk loop keeps no register for it.  A wave makes at
+ * most 64 deposits per step, so at most one multiple is crossed per step. */
+#define TMC_SLOT_FLUSH_DEPOSITS (1ull << 20)
+/* Most wave steps between two flushes (a step deposits at least once): the
+ * horizon of the absorption-cap test. */
+#define TMC_SLOT_FLUSH_STEPS (uint32_t)TMC_SLOT_FLUSH_DEPOSITS
 
 /* 64-bit counter words of a batched launch. */
 #define TMC_BATCH_COUNTERS 8

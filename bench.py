@@ -42,6 +42,26 @@ FLOP_PER_INTERACTION = 40.5  # FP32 FLOPs of photon.c:21-48 per absorption event
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "walk_f32_default_2e30_traffic.txt")
 REF_BENCH = os.path.join(ROOT, "oracle", "_ref", "tiny_mc_bench")
+REF_SHIPPED = os.path.join(ROOT, "oracle", "_ref", "tiny_mc_shipped")
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def ref_rate(exe, seed, photons):
+    """K photons/s line of one run of a reference build (tiny_mc.c:53-56)."""
+    env = dict(os.environ, TMC_SEED=str(seed), TMC_PHOTONS=str(photons))
+    out = subprocess.run([exe], env=env, stdout=subprocess.PIPE, text=True, check=True).stdout
+    m = re.search(r"^# ([0-9.]+) K photons per second", out, re.M)
+    return float(m.group(1)) * 1e3 if m else None
 
 
 def progress(msg):
@@ -74,12 +94,15 @@ def cpu_baseline(max_procs=16, photons_per_proc=1 << 18):
         if m:
             rates.append(float(m.group(1)) * 1e3)
     wall = time.perf_counter() - t0
+    shipped = ref_rate(REF_SHIPPED, 1, photons_per_proc // 2) if os.path.exists(REF_SHIPPED) else None
     return {
         "value": procs * photons_per_proc / wall,
         "unit": "photons/s",
         "cores": procs,
         "kind": kind,
+        "cpu_model": cpu_model(),
         "single_core_value": sum(rates) / len(rates) if rates else None,
+        "single_core_as_shipped_value": shipped,
         "sample": f"{procs} processes x {photons_per_proc} photons, default physics, seeds 1..{procs}; "
                   + ("unmodified reference tiny_mc.c/photon.c/wtime.c built -std=c11 -O3 -march=x86-64-v3"
                      if kind == "reference" else "oracle restatement of the reference loop (oracle/liboracle.so)"),

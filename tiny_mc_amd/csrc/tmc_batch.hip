@@ -170,7 +170,6 @@ __device__ __forceinline__ void load_set(SetState& st, const BatchArgs& a, uint3
     st.phys.spm = __uint_as_float((uint32_t)(w[3] >> 32));
     st.phys.shell_max = __uint_as_float((uint32_t)w[4]);
     st.shells = (uint32_t)(w[4] >> 32);
-    st.phys.last_shell = st.shells - 1;
     set_derived<batch_keys<SAMPLER, SERVE>()>(st.phys);
     st.kcap = (uint32_t)w[5];
     st.done = a.done + 2 * (uint64_t)index;

@@ -24,8 +24,7 @@ struct Photon {
 struct Physics {
     float albedo; /* mu_s / (mu_s + mu_a) in binary32 (photon.c:8) */
     float spm; /* shells per mean free path (photon.c:9) */
-    float shell_max; /* (float)(shells - 1) */
-    uint32_t last_shell; /* shells - 1 */
+    float shell_max; /* (float)(shells - 1), the overflow shell */
     uint32_t key0, key1; /* Philox key = 64-bit seed */
     float far_min; /* shell_max + 1, set by set_derived */
     float clear_half; /* 0.5 - eps, set by set_derived */
@@ -78,8 +77,8 @@ __device__ __forceinline__ float shell_coord_short(float r2, const Physics& ph, 
  * the same floor and the same side of shell_max when it is clear of every
  * integer by eps or lies at or beyond shell_max + 1 (checked on every float
  * r2 by tiny_mc_gpu_check_shell_index).  Otherwise, for a lane near a shell
- * boundary (a few per ten thousand at 101 shells) or a radius below about
- * 2^-48, the whole wave takes the IEEE square root. */
+ * boundary (a few per ten thousand at 101 shells, including radii within
+ * eps / spm of the origin), the whole wave takes the IEEE square root. */
 __device__ __forceinline__ uint32_t shell_index(float r2, const Physics& ph, bool* inner)
 {
     bool sure;
@@ -142,15 +141,14 @@ __device__ __forceinline__ bool cap_within(const PhotonT& ph, uint32_t kcap, uin
 #define TMC_ROULETTE_KILL_R 0x19999900u
 
 /* Walks one absorption event of p, adding its deposit to the fixed-point
- * shell sums in LDS unless deposit is false.  Returns false when the
- * roulette kills the photon.
+ * shell sums in LDS.  Returns false when the roulette kills the photon.
  * MATH (TMC_MATH_*) selects how -ln u, sin/cos(2 pi u') and the analytic
  * sampler's sin(theta) are evaluated; everything else is the same
  * arithmetic in both modes.  STORED_KEYS takes the Philox round keys from
  * ph.keys; otherwise they are formed from the seed inside the step (fewer
  * live registers, for kernels short of scalar registers). */
 template <uint32_t SAMPLER, uint32_t MATH, bool STORED_KEYS = true>
-__device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt, bool deposit = true)
+__device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt)
 {
     const tmc_u32x4 r = STORED_KEYS ? tmc_philox4x32_10_block0_ks(p.k, p.pp, ph.keys) /* block (k, 0) */
                                     : tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1);
@@ -163,10 +161,9 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
 
     /* absorb (photon.c:26-32) */
     const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
-    bool inner; /* else the overflow shell, ph.last_shell */
+    bool inner; /* else the overflow shell */
     const uint32_t shell = shell_index(r2, ph, &inner);
-    if (deposit)
-        tally_add(lt, inner, shell, fx_heat(p.weight), fx_heat2(p.weight));
+    tally_add(lt, inner, shell, fx_heat(p.weight), fx_heat2(p.weight));
     p.weight = p.weight * ph.albedo;
 
     /* new direction (photon.c:34-43) */

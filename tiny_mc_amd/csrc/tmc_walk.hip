@@ -200,7 +200,6 @@ __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
     phys.albedo = a.albedo;
     phys.spm = a.spm;
     phys.shell_max = a.shell_max;
-    phys.last_shell = a.shells - 1;
     phys.key0 = (uint32_t)a.seed;
     phys.key1 = (uint32_t)(a.seed >> 32);
     set_derived(phys);

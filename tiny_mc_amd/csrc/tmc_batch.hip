@@ -163,7 +163,7 @@ __device__ __forceinline__ void load_set(SetState& st, const BatchArgs& a, uint3
     }
     st.phys.key0 = (uint32_t)w[0];
     st.phys.key1 = (uint32_t)(w[0] >> 32);
-    st.first_photon = w[1];
+    st.first_photon = rfl64(w[1]); /* scalar: the base of every photon launch */
     st.tally_offset = w[2];
     st.tally = a.tally + w[2];
     st.phys.albedo = __uint_as_float((uint32_t)w[3]);
@@ -281,13 +281,16 @@ __device__ __forceinline__ void finish_request(const BatchArgs& a, const SetStat
  * since the last publication (and their absorptions) to the set, and the
  * absorptions and idle lane-steps (batch) of the steps since then to the
  * launch counters (published here rather than summed per step, to keep the
- * walk loop's scalar registers few).  Serving: the sums and
+ * walk loop's scalar registers few).  A wave publishes at every flush, so
+ * between two publications it makes fewer than TMC_SLOT_FLUSH_DEPOSITS + 64
+ * absorptions, as many steps at most, and at most 64 completions per step:
+ * 32-bit counts.  Serving: the sums and
  * the counters are system-scope atomics, and the wave waits for every sum
  * update to complete before it adds to the counters, so the wave whose
  * count completes the request finds final sums. */
 template <bool SERVE>
 __device__ __forceinline__ void flush_and_publish(unsigned long long* slot, const BatchArgs& a, const SetState& st,
-                                                  uint32_t lane, uint64_t& completed, uint64_t& absorbed,
+                                                  uint32_t lane, uint32_t& completed, uint32_t& absorbed,
                                                   uint32_t& steps)
 {
     flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
@@ -341,8 +344,9 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     enum : uint32_t { PENDING = 1u, QUEUE_DONE = 2u, CAPPING = 4u /* per-step cap checks until the next flush */ };
     uint32_t flags = 0;
     BatchChunk pend = { 0u, 0u, 0ull };
-    uint64_t completed = 0, absorbed = 0;
-    uint32_t steps = 0; /* steps since the last publication */
+    /* photons completed, absorptions and wave steps since the last
+     * publication (32-bit: see flush_and_publish) */
+    uint32_t completed = 0, absorbed = 0, steps = 0;
     uint64_t stepped = 0; /* lanes that walked in the previous iteration */
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
     uint32_t poll_wait = 0; /* serving: steps before re-polling an unpublished ticket */
@@ -356,7 +360,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     for (;;) {
         uint64_t dead = __ballot(ph.weight < 0);
         /* lanes whose photon ended in the previous step (roulette or cap) */
-        completed += (uint64_t)__popcll(dead & stepped);
+        completed += (uint32_t)__popcll(dead & stepped);
         stepped = 0;
         if (dead != 0) {
             if (SERVE && poll_wait && dead != ~0ull) {
@@ -400,21 +404,19 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 pool_end = pend.first + pend.count;
                 flags &= ~PENDING;
             }
-            const uint64_t avail = pool_end - pool_next;
-            if (avail) {
-                const uint32_t ndead = (uint32_t)__popcll(dead);
-                const uint32_t rank = mask_rank(dead);
-                if (ph.weight < 0 && rank < avail)
-                    launch_photon(ph, st.first_photon + pool_next + rank, st.phys.key0, st.phys.key1);
-                pool_next += avail < ndead ? avail : ndead;
+            if (pool_next != pool_end) {
+                const uint32_t take = pool_take(pool_next, pool_end, (uint32_t)__popcll(dead));
+                const uint64_t first = st.first_photon + pool_next;
+                if (ph.weight < 0 && mask_rank(dead) < take)
+                    launch_photon(ph, first + mask_rank(dead), st.phys.key0, st.phys.key1);
+                pool_next += take;
             }
         }
         const uint64_t live = __ballot(ph.weight >= 0);
         if (live == 0 && (flags & (PENDING | QUEUE_DONE)) == QUEUE_DONE)
             break;
         if (live != 0) {
-            const uint64_t absorbed_before = absorbed;
-            absorbed += (uint64_t)__popcll(live);
+            absorbed += (uint32_t)__popcll(live);
             ++steps;
 
             /* the loop has one back-edge (no continue): with several, the
@@ -435,14 +437,14 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             }
             stepped = live;
 
-            /* overflow guard: a flush whenever the absorptions since the last
-             * publication (which flushes too) cross a multiple of
-             * TMC_SLOT_FLUSH_DEPOSITS, so no slot word takes more deposits
-             * between two flushes; completions are published on switch /
-             * idle.  The cap condition is evaluated at least as often: the
-             * other publications happen only when every lane is empty. */
-            if (((absorbed_before ^ absorbed) / TMC_SLOT_FLUSH_DEPOSITS) != 0) {
-                flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
+            /* overflow guard: a flush (with a publication) once the
+             * absorptions since the last publication (which flushes too)
+             * reach TMC_SLOT_FLUSH_DEPOSITS, so no slot word takes more
+             * deposits between two flushes.  The cap condition is evaluated
+             * at least as often: the other publications happen only when
+             * every lane is empty. */
+            if (absorbed >= (uint32_t)TMC_SLOT_FLUSH_DEPOSITS) {
+                flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
                 flags = cap_within(ph, st.kcap, TMC_SLOT_FLUSH_STEPS) ? (flags | CAPPING) : (flags & ~CAPPING);
             }
         }

@@ -43,6 +43,17 @@ __device__ __forceinline__ void add128_sys(unsigned long long* p, unsigned long 
  * maximum). */
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+/* Photons a wave takes from its pool [next, end) of one claimed chunk for
+ * its n dead lanes: min(end - next, n).  A chunk holds fewer than 2^32
+ * photons, so the count is formed in 32 bits (one scalar min; the scalar
+ * unit has no 64-bit ordered compare, so a 64-bit min of wave-uniform
+ * values would run on the vector unit). */
+__device__ __forceinline__ uint32_t pool_take(uint64_t next, uint64_t end, uint32_t n)
+{
+    const uint32_t avail = (uint32_t)(end - next);
+    return avail < n ? avail : n;
+}
+
 /* Rank of this lane among the lanes set in `mask` (lanes below it). */
 __device__ __forceinline__ uint32_t mask_rank(uint64_t mask)
 {

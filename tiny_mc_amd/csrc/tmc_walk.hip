@@ -121,41 +121,28 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
     for (;;) {
         const uint64_t dead = __ballot(ph.weight < 0);
         if (dead != 0 && !exhausted) {
-            const uint32_t ndead = (uint32_t)__popcll(dead);
-            const uint32_t rank = mask_rank(dead);
-            const uint64_t avail = pool_end - pool_next;
-            uint64_t nb = 0, ne = 0;
-            if (avail < ndead) {
+            /* the dead lanes take photons from the pool; an empty pool is
+             * refilled with the next chunk first.  Lanes the pool cannot
+             * serve wait for the next step's claim, so every launch indexes
+             * one contiguous range (pool_next + rank). */
+            if (pool_next == pool_end) {
                 unsigned long long base = 0;
                 if (lane == 0)
                     base = atomicAdd(a.counters, (unsigned long long)a.chunk);
                 base = rfl64(base);
                 if (base < a.photons) {
-                    nb = base;
-                    ne = base + a.chunk < a.photons ? base + a.chunk : a.photons;
+                    pool_next = base;
+                    pool_end = base + a.chunk < a.photons ? base + a.chunk : a.photons;
                 } else {
                     exhausted = true;
                 }
             }
-            if (ph.weight < 0) {
-                bool got = false;
-                uint64_t idx = 0;
-                if (rank < avail) {
-                    idx = pool_next + rank;
-                    got = true;
-                } else if (rank - avail < ne - nb) {
-                    idx = nb + (rank - avail);
-                    got = true;
-                }
-                if (got)
-                    launch_photon(ph, a.first_photon + idx, phys.key0, phys.key1);
-            }
-            if (avail >= ndead) {
-                pool_next += ndead;
-            } else {
-                const uint64_t take = ndead - avail;
-                pool_next = nb + (take < ne - nb ? take : ne - nb);
-                pool_end = ne;
+            if (!exhausted) {
+                const uint32_t take = pool_take(pool_next, pool_end, (uint32_t)__popcll(dead));
+                const uint64_t first = a.first_photon + pool_next;
+                if (ph.weight < 0 && mask_rank(dead) < take)
+                    launch_photon(ph, first + mask_rank(dead), phys.key0, phys.key1);
+                pool_next += take;
             }
         }
         const uint64_t live = __ballot(ph.weight >= 0);

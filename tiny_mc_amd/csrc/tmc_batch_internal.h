@@ -75,12 +75,11 @@ struct BatchArgs {
     uint32_t refresh_ticks; /* realtime ticks (100 MHz) between reads of the host control words */
 };
 
-/* Deposits between flushes of a wave's LDS slot: a wave alone adds to its
- * slot, so 2^20 deposits of at most 2^43 (the heat2 image of w = 1) per
- * entry stay below 2^63.  A power of two (the walk detects a crossing of its
- * multiples in the running absorption count) and a constant rather than a
- * kernel argument: the walk loop keeps no register for it.  A wave makes at
- * most 64 deposits per step, so at most one multiple is crossed per step. */
+/* Deposits between flushes of a wave's LDS slot: the wave flushes once its
+ * absorptions since the last flush reach this count, so a slot word takes
+ * fewer than 2^20 + 64 deposits (a wave makes at most 64 per step) of at
+ * most 2^43 each (the heat2 image of w = 1): below 2^64.  A constant rather
+ * than a kernel argument: the walk loop keeps no register for it. */
 #define TMC_SLOT_FLUSH_DEPOSITS (1ull << 20)
 /* Most wave steps between two flushes (a step deposits at least once): the
  * horizon of the absorption-cap test. */

@@ -290,7 +290,7 @@ typedef struct tmc_serve_options {
     uint64_t tally_words; /* device tally arena in 64-bit words; 0 = max_requests * 4 * max_shells */
     uint64_t ring_chunks; /* chunk ring entries, power of two; 0 = 2^20 */
     uint64_t chunk_work; /* target absorptions per chunk; 0 = 2^17 */
-    uint32_t max_photons_per_lane; /* cap on photons per lane in a chunk; 0 = 16 */
+    uint32_t max_photons_per_lane; /* cap on photons per lane in a chunk, <= 2^24; 0 = 16 */
     uint32_t idle_ms; /* the server stops the kernel after this long without requests; 0 = 50 */
     uint32_t math; /* TMC_MATH_* */
     uint32_t reserved; /* 0 */

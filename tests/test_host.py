@@ -7,7 +7,8 @@
 - the sweep and serving workloads are the stated distributions, and the
   sweep is pinned (its total photon budget at n=4096, seed 0);
 - the batch tally layout (tmc_batch_tally_words, split_tallies) is the
-  concatenation of 4 * shells words per set.
+  concatenation of 4 * shells words per set;
+- tmc_serve_open rejects invalid options before any device work.
 """
 import ctypes
 
@@ -97,3 +98,16 @@ def test_batch_tally_layout(tmc):
     assert [len(x) for x in parts] == [4 * q.shells for q, _ in sets]
     assert np.array_equal(np.concatenate(parts), flat)
     assert lib.tmc_batch_tally_words(None, 3) == 0
+
+
+@pytest.mark.parametrize("kw", [dict(sampler=2), dict(math=2), dict(max_shells=8193), dict(block=96),
+                                dict(block=1088), dict(ring_chunks=1000), dict(max_requests=65537),
+                                dict(max_photons_per_lane=(1 << 24) + 1)])
+def test_serve_open_rejects_invalid_options(tmc, kw):
+    """Invalid serving options fail with TMC_E_INVALID before the server
+    touches a device, so on any host.  The per-lane chunk cap keeps a chunk's
+    photon count (at most 64 * max_photons_per_lane) within its 32-bit
+    field."""
+    with pytest.raises(tmc.TmcError) as e:
+        tmc.Server(**kw)
+    assert e.value.status == -1

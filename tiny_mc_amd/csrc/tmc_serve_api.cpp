@@ -438,8 +438,11 @@ TMC_EXPORT int tmc_serve_open(const tmc_serve_options* opt, tmc_server** out)
         o.idle_ms = 50;
     if (!o.tally_words)
         o.tally_words = (uint64_t)o.max_requests * 4 * o.max_shells;
+    /* a chunk (at most 64 * max_photons_per_lane photons) keeps its photon
+     * count in 32 bits */
     if (o.max_shells > TMC_MAX_SHELLS || o.max_requests > (1u << TMC_SERVE_SLOT_BITS) || o.block % 64
-        || o.block > 1024 || (o.ring_chunks & (o.ring_chunks - 1)) || o.ring_chunks > (1ull << 31))
+        || o.block > 1024 || (o.ring_chunks & (o.ring_chunks - 1)) || o.ring_chunks > (1ull << 31)
+        || o.max_photons_per_lane > (1u << 24))
         return TMC_E_INVALID;
 
     std::lock_guard<std::mutex> lock(tmc::api_mutex());

@@ -42,7 +42,7 @@ typedef enum tmc_status {
     TMC_E_NODEVICE = -2, /* no HIP device usable */
     TMC_E_HIP = -3, /* a HIP runtime call failed */
     TMC_E_NOMEM = -4, /* device or host allocation failed */
-    TMC_E_RANGE = -5, /* photon index range overflows 64 bits */
+    TMC_E_RANGE = -5, /* photon index range first_photon + photons exceeds 2^63 */
     TMC_E_UNSUPPORTED = -6 /* option or library build not available */
 } tmc_status;
 

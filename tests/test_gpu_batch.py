@@ -72,6 +72,12 @@ def test_first_photon_offsets_and_empty_sets(tmc):
     np.testing.assert_array_equal(parts[4], tmc.run_raw(p, 3, first_photon=(1 << 32) - 2500)[0])
     flat0, st0 = tmc.batch_run([])
     assert flat0.size == 0 and st0.photons == 0
+    # photon index ranges end at 2^63 at most, as in a single run
+    far = [(p, 3, (1 << 63) - 5000)]
+    np.testing.assert_array_equal(tmc.batch_run(far)[0], tmc.run_raw(p, 3, first_photon=(1 << 63) - 5000)[0])
+    with pytest.raises(tmc.TmcError) as e:
+        tmc.batch_run([(p, 3, (1 << 63) - 4999)])
+    assert e.value.status == -5
 
 
 def test_graph_replays_identical(tmc):

@@ -65,6 +65,9 @@ def test_zero_photons_bad_params_and_poll(tmc):
         assert not t.any() and rs.photons == 0
         with pytest.raises(tmc.TmcError):
             srv.submit(tmc.Params(mu_a=0.0), 1)
+        with pytest.raises(tmc.TmcError) as e:
+            srv.submit(tmc.Params(photons=10), 1, first_photon=(1 << 63) - 5)
+        assert e.value.status == -5
         rid = srv.submit(tmc.Params(photons=200000), 7)
         res = None
         for _ in range(20000):

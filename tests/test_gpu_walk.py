@@ -114,9 +114,13 @@ def test_zero_photons_and_bad_params(tmc):
         with pytest.raises(tmc.TmcError) as e:
             tmc.run_raw(_params(tmc, bad, 10), 1)
         assert e.value.status == -1
-    with pytest.raises(tmc.TmcError) as e:
-        tmc.run_raw(_params(tmc, DEFAULT, 10), 1, first_photon=(1 << 64) - 5)
-    assert e.value.status == -5
+    # photon index ranges end at 2^63 at most
+    for first in ((1 << 64) - 5, (1 << 63) - 5):
+        with pytest.raises(tmc.TmcError) as e:
+            tmc.run_raw(_params(tmc, DEFAULT, 10), 1, first_photon=first)
+        assert e.value.status == -5
+    t, st = tmc.run_raw(_params(tmc, DEFAULT, 10), 1, first_photon=(1 << 63) - 10)
+    assert st.photons == 10 and t.any()
 
 
 def test_heat_api_matches_tally(tmc):

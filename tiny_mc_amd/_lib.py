@@ -32,7 +32,7 @@ _STATUS = {
     -2: "no usable HIP device",
     -3: "HIP runtime error",
     -4: "out of memory",
-    -5: "photon index range overflows 64 bits",
+    -5: "photon index range exceeds 2^63",
     -6: "unsupported option",
 }
 

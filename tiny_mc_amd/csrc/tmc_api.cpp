@@ -210,7 +210,7 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     const uint32_t S = p->shells;
     if (o->sampler > TMC_SAMPLER_REJECTION || o->precision > TMC_PRECISION_FP64 || o->math > TMC_MATH_OWN)
         return TMC_E_INVALID;
-    if (p->photons > UINT64_MAX - o->first_photon)
+    if (!tmc::photon_range_ok(o->first_photon, p->photons))
         return TMC_E_RANGE;
 
     uint32_t block = o->block ? o->block : 256u;
@@ -422,7 +422,7 @@ TMC_EXPORT const char* tiny_mc_gpu_strerror(int status)
     case TMC_E_NODEVICE: return "no usable HIP device";
     case TMC_E_HIP: return "HIP runtime error";
     case TMC_E_NOMEM: return "out of memory";
-    case TMC_E_RANGE: return "photon index range overflows 64 bits";
+    case TMC_E_RANGE: return "photon index range exceeds 2^63";
     case TMC_E_UNSUPPORTED: return "unsupported option";
     case TMC_PENDING: return "pending";
     default: return "unknown status";

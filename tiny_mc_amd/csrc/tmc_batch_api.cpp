@@ -436,7 +436,7 @@ int check_sets(const tmc_set* sets, uint32_t nsets)
         const int rc = tmc::check_params(&sets[i].params);
         if (rc != TMC_OK)
             return rc;
-        if (sets[i].params.photons > UINT64_MAX - sets[i].first_photon)
+        if (!tmc::photon_range_ok(sets[i].first_photon, sets[i].params.photons))
             return TMC_E_RANGE;
     }
     return TMC_OK;

@@ -33,6 +33,15 @@ int acquire_context(Context** out);
  * TMC_E_INVALID). */
 int check_params(const tmc_params* p);
 
+/* Whether the photon index range [first, first + photons) lies below 2^63:
+ * then no photon index, chunk end or claim counter (which runs past the end
+ * of the range by at most one chunk per wave) wraps around 64 bits. */
+inline bool photon_range_ok(uint64_t first, uint64_t photons)
+{
+    const uint64_t limit = 1ull << 63;
+    return first <= limit && photons <= limit - first;
+}
+
 /* photon.c:8: albedo = mu_s / (mu_s + mu_a) in binary32. */
 float derived_albedo(const tmc_params* p);
 

@@ -529,7 +529,7 @@ TMC_EXPORT int tmc_serve_submit(tmc_server* s, const tmc_set* req, uint64_t* id)
         return rc;
     if (req->params.shells > s->opt.max_shells)
         return TMC_E_INVALID;
-    if (req->params.photons > UINT64_MAX - req->first_photon)
+    if (!tmc::photon_range_ok(req->first_photon, req->params.photons))
         return TMC_E_RANGE;
     const uint64_t words = 4 * (uint64_t)req->params.shells;
     if (words > s->opt.tally_words)

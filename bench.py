@@ -69,7 +69,7 @@ def progress(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(max_procs=16, photons_per_proc=1 << 18):
+def cpu_baseline(max_procs=16, photons_per_proc=1 << 19):
     """The reference loop (photon() + glibc rand(), unmodified sources built
     by oracle/Makefile) on the host cores: one process per core, distinct
     seeds (threads would serialise on rand()'s lock).  Σ photons / wall."""

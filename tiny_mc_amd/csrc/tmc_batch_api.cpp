@@ -297,35 +297,8 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
             items[g].push_back({ { i, (uint32_t)cnt, first }, (double)((cnt + 63) / 64) * L[i] });
         }
     }
-    for (size_t g = 0; g < items.size(); ++g) {
-        auto& v = items[g];
+    for (auto& v : items)
         std::stable_sort(v.begin(), v.end(), [](const Item& x, const Item& y) { return x.dur > y.dur; });
-        /* the chunks claimed last decide when the pass ends: those of the
-         * last 2 x resident-waves queue positions are cut into quarters
-         * (whole multiples of 64 photons) */
-        const size_t waves = (size_t)groups[g].shape.grid * (groups[g].shape.block / 64u);
-        const size_t tail = std::min(v.size(), 2 * waves);
-        std::vector<Item> cut;
-        for (size_t k = v.size() - tail; k < v.size(); ++k) {
-            const Item it = v[k];
-            const uint64_t per_lane = (it.ch.count + 63u) / 64u;
-            const uint64_t parts = std::min<uint64_t>(4u, per_lane);
-            const uint64_t piece = (per_lane / parts) * 64u;
-            uint64_t first = it.ch.first, left = it.ch.count;
-            for (uint64_t q = 0; q < parts; ++q) {
-                const uint64_t cnt = q + 1 == parts ? left : std::min<uint64_t>(piece, left);
-                if (!cnt)
-                    break;
-                const double L = it.dur / (double)per_lane;
-                cut.push_back({ { it.ch.set, (uint32_t)cnt, first }, (double)((cnt + 63) / 64) * L });
-                first += cnt;
-                left -= cnt;
-            }
-        }
-        v.resize(v.size() - tail);
-        std::stable_sort(cut.begin(), cut.end(), [](const Item& x, const Item& y) { return x.dur > y.dur; });
-        v.insert(v.end(), cut.begin(), cut.end());
-    }
     /* the pass with the longest chunk first, so that the launch sequence ends
      * on short chunks */
     std::vector<size_t> order(groups.size());

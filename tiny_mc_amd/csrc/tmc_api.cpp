@@ -8,6 +8,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -216,9 +217,12 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     uint32_t block = o->block ? o->block : 256u;
     if (block < 64u || block > 1024u || (block % 64u) != 0u)
         return TMC_E_INVALID;
-    uint32_t chunk = o->chunk ? o->chunk : 256u;
-    if (chunk < 64u || (chunk % 64u) != 0u || chunk > (1u << 24))
+    uint32_t chunk = o->chunk;
+    if (chunk && (chunk < 64u || (chunk % 64u) != 0u || chunk > (1u << 24)))
         return TMC_E_INVALID;
+    uint64_t kcap = o->max_interactions ? o->max_interactions : 0xFFFFFFFFull;
+    if (kcap > 0xFFFFFFFFull)
+        kcap = 0xFFFFFFFFull;
     uint32_t grid = o->grid;
     if (grid == 0) {
         int per_cu = 0;
@@ -232,6 +236,20 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     const uint64_t wg_needed = (waves_needed + waves_per_wg - 1) / waves_per_wg;
     if (wg_needed < grid)
         grid = (uint32_t)(wg_needed ? wg_needed : 1u);
+    if (!chunk) {
+        /* photons a wave claims at a time (64 per lane-photon): about 256
+         * expected absorptions per lane between two claims, as a wave waits
+         * for its claim's device-scope atomic (short walks would otherwise
+         * claim every few steps, all waves on one counter), at least 4 per
+         * lane, and few enough that each wave gets about 4 claims, so the
+         * waves finish together */
+        const double L = std::max(1.0, tmc::expected_interactions(p, kcap));
+        const double waves = (double)grid * waves_per_wg;
+        double per_lane = std::ceil(256.0 / L);
+        per_lane = std::min(per_lane, std::floor((double)p->photons / (64.0 * 4.0 * waves)));
+        per_lane = std::min(std::max(per_lane, 4.0), (double)(1u << 18));
+        chunk = 64u * (uint32_t)per_lane;
+    }
 
     int rc = ensure_tally(c, 4 * (size_t)S);
     if (rc != TMC_OK)
@@ -262,9 +280,6 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
      * waves_per_wg * 64 * flush_interval deposits of at most 2^43 (heat2
      * image of w = 1) accumulate: keep that below 2^63. */
     a.flush_interval = (1u << 14) / waves_per_wg;
-    uint64_t kcap = o->max_interactions ? o->max_interactions : 0xFFFFFFFFull;
-    if (kcap > 0xFFFFFFFFull)
-        kcap = 0xFFFFFFFFull;
     a.kcap = (uint32_t)kcap;
     a.counters = c->d_counters;
     a.tally = c->d_tally;

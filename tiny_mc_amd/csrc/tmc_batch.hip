@@ -357,15 +357,22 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     launch_photon(ph, 0, 0u, 0u);
     ph.weight = -1;
 
-    for (;;) {
+    bool more = true;
+    while (more) {
         uint64_t dead = __ballot(ph.weight < 0);
         /* lanes whose photon ended in the previous step (roulette or cap) */
         completed += (uint32_t)__popcll(dead & stepped);
         stepped = 0;
-        if (dead != 0) {
+        /* conditions are tested one at a time, the common case first: the
+         * compiler turned the combined forms into chains of scalar selects
+         * evaluated on every step */
+        if (dead != 0 && pool_next == pool_end) {
+            /* the pool is empty: claim a chunk, or switch sets once drained
+             * (PENDING is set by a claim with the pool empty and stays so
+             * until the switch) */
             if (SERVE && poll_wait && dead != ~0ull) {
                 --poll_wait; /* live lanes keep walking; poll again later */
-            } else if (pool_next == pool_end && !(flags & (PENDING | QUEUE_DONE))) {
+            } else if (!(flags & (PENDING | QUEUE_DONE))) {
                 BatchChunk ch;
                 const ClaimResult r = try_claim<SERVE>(a, lane, ticket, ch);
                 if (r == CLAIM_GOT) {
@@ -404,6 +411,8 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 pool_end = pend.first + pend.count;
                 flags &= ~PENDING;
             }
+        }
+        if (dead != 0) {
             if (pool_next != pool_end) {
                 const uint32_t take = pool_take(pool_next, pool_end, (uint32_t)__popcll(dead));
                 const uint64_t first = st.first_photon + pool_next;
@@ -413,8 +422,12 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             }
         }
         const uint64_t live = __ballot(ph.weight >= 0);
-        if (live == 0 && (flags & (PENDING | QUEUE_DONE)) == QUEUE_DONE)
-            break;
+        /* the loop ends with no photon left and no chunk to come (a loop
+         * condition rather than a break: with a break, or with this
+         * assignment in an else branch of the step, the compiler tested the
+         * exit through chains of scalar selects, or copied the photon state
+         * around the step) */
+        more = live != 0 || (flags & (PENDING | QUEUE_DONE)) != QUEUE_DONE;
         if (live != 0) {
             absorbed += (uint32_t)__popcll(live);
             ++steps;

@@ -283,8 +283,8 @@ __device__ __forceinline__ void finish_request(const BatchArgs& a, const SetStat
  * launch counters (published here rather than summed per step, to keep the
  * walk loop's scalar registers few).  A wave publishes at every flush, so
  * between two publications it makes fewer than TMC_SLOT_FLUSH_DEPOSITS + 64
- * absorptions, as many steps at most, and at most 64 completions per step:
- * 32-bit counts.  Serving: the sums and
+ * absorptions, as many steps at most, and completes at most as many photons
+ * as it launched plus 64: 32-bit counts.  Serving: the sums and
  * the counters are system-scope atomics, and the wave waits for every sum
  * update to complete before it adds to the counters, so the wave whose
  * count completes the request finds final sums. */
@@ -344,10 +344,12 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     enum : uint32_t { PENDING = 1u, QUEUE_DONE = 2u, CAPPING = 4u /* per-step cap checks until the next flush */ };
     uint32_t flags = 0;
     BatchChunk pend = { 0u, 0u, 0ull };
-    /* photons completed, absorptions and wave steps since the last
-     * publication (32-bit: see flush_and_publish) */
-    uint32_t completed = 0, absorbed = 0, steps = 0;
-    uint64_t stepped = 0; /* lanes that walked in the previous iteration */
+    /* absorptions, wave steps and photons launched since the last
+     * publication (32-bit: see flush_and_publish), and the lanes live at it:
+     * the photons completed in between are launched + live_pub - live now,
+     * formed only when they are published (a per-step count of the lanes
+     * whose photon ended cost scalar instructions on every step) */
+    uint32_t absorbed = 0, steps = 0, launched = 0, live_pub = 0;
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
     uint32_t poll_wait = 0; /* serving: steps before re-polling an unpublished ticket */
 
@@ -360,9 +362,6 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     bool more = true;
     while (more) {
         uint64_t dead = __ballot(ph.weight < 0);
-        /* lanes whose photon ended in the previous step (roulette or cap) */
-        completed += (uint32_t)__popcll(dead & stepped);
-        stepped = 0;
         /* conditions are tested one at a time, the common case first: the
          * compiler turned the combined forms into chains of scalar selects
          * evaluated on every step */
@@ -394,15 +393,21 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                      * The host asks the launch to stop only once nothing is
                      * active, when every outstanding ticket lies past the
                      * tail. */
-                    if (cur_set != 0xFFFFFFFFu)
+                    if (cur_set != 0xFFFFFFFFu) {
+                        uint32_t completed = launched + live_pub; /* no lane is live */
                         flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
+                        launched = live_pub = 0;
+                    }
                     __builtin_amdgcn_s_sleep(64);
                 }
             }
             if ((flags & PENDING) && dead == ~0ull) {
                 /* drained: hand the finished set's sums over, switch sets */
-                if (cur_set != 0xFFFFFFFFu)
+                if (cur_set != 0xFFFFFFFFu) {
+                    uint32_t completed = launched + live_pub; /* no lane is live */
                     flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
+                    launched = live_pub = 0;
+                }
                 cur_set = pend.set;
                 load_set<SAMPLER, SERVE>(st, a, cur_set & slot_mask, lane);
                 /* every lane is empty here: only a small cap needs checks */
@@ -419,6 +424,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 if (ph.weight < 0 && mask_rank(dead) < take)
                     launch_photon(ph, first + mask_rank(dead), st.phys.key0, st.phys.key1);
                 pool_next += take;
+                launched += take;
             }
         }
         const uint64_t live = __ballot(ph.weight >= 0);
@@ -448,7 +454,6 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                         ph.weight = -1;
                 }
             }
-            stepped = live;
 
             /* overflow guard: a flush (with a publication) once the
              * absorptions since the last publication (which flushes too)
@@ -457,13 +462,19 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
              * at least as often: the other publications happen only when
              * every lane is empty. */
             if (absorbed >= (uint32_t)TMC_SLOT_FLUSH_DEPOSITS) {
+                const uint32_t live_now = (uint32_t)__popcll(__ballot(ph.weight >= 0));
+                uint32_t completed = launched + live_pub - live_now;
                 flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
+                launched = 0;
+                live_pub = live_now;
                 flags = cap_within(ph, st.kcap, TMC_SLOT_FLUSH_STEPS) ? (flags | CAPPING) : (flags & ~CAPPING);
             }
         }
     }
-    if (cur_set != 0xFFFFFFFFu)
+    if (cur_set != 0xFFFFFFFFu) {
+        uint32_t completed = launched + live_pub; /* no lane is live */
         flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
+    }
     if (lane == 0) {
         /* serving: the last wave to leave rewinds the ticket counter for the
          * next launch (no wave takes a ticket after it has left) */

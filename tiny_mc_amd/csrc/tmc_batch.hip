@@ -12,9 +12,10 @@
  * Tallies: every wave owns an LDS slot of shells_max shells (1 or 2 copies,
  * lanes 0-31 / 32-63 in separate copies to halve same-address collisions),
  * flushed into the set's 128-bit global sums when the wave changes set,
- * every TMC_SLOT_FLUSH_DEPOSITS absorptions, and at exit.  After each flush the wave adds
- * the photons it completed for that set to the set's done counter (release
- * ordering), so done == photons means the set's tallies are complete.
+ * every TMC_SLOT_FLUSH_DEPOSITS absorptions, and at exit.  With each flush
+ * the wave adds the photons it completed for that set to the set's done
+ * counter (serving: once its sum updates have completed), so done ==
+ * photons means the set's tallies are complete.
  *
  * The per-absorption arithmetic is the same step<>() as the single-run
  * kernel, so each set's tallies are bit-identical to a single run of that

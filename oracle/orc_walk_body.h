@@ -31,6 +31,7 @@
 static uint64_t W_FN(W_SRC* src, W_SINK* sink, W_REAL albedo, W_REAL shells_per_mfp, uint32_t shells,
                      int sampler, uint64_t max_k)
 {
+    (void)src; /* the glibc rand() instantiation draws from the global stream */
     W_REAL x = 0, y = 0, z = 0;
     W_REAL u = 0, v = 0, w = 1;
     W_REAL weight = 1;

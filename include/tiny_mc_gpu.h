@@ -43,7 +43,10 @@ typedef enum tmc_status {
     TMC_E_HIP = -3, /* a HIP runtime call failed */
     TMC_E_NOMEM = -4, /* device or host allocation failed */
     TMC_E_RANGE = -5, /* photon index range first_photon + photons exceeds 2^63 */
-    TMC_E_UNSUPPORTED = -6 /* option or library build not available */
+    TMC_E_UNSUPPORTED = -6, /* option or library build not available */
+    TMC_E_CANCELLED = -8, /* serving: the request was cancelled (tmc_serve_cancel, tmc_serve_close_timeout) */
+    TMC_E_DEADLINE = -9, /* serving: the request's deadline passed before it finished */
+    TMC_E_TIMEOUT = -10 /* serving: tmc_serve_close_timeout ran out of time; the server is still open */
 } tmc_status;
 
 /* Physics and size of one simulation; the fields are the macros of
@@ -277,7 +280,12 @@ int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc_batch_opti
  * result is bit-identical to tiny_mc_gpu_run_ex of the same set alone
  * (binary32 arithmetic, TMC_PRECISION_FP32).  A server owns a host thread
  * and one default-priority stream on the context's device; while it holds
- * requests its kernel occupies all but a few workgroup slots of the device.
+ * requests its kernel occupies the device's workgroup slots but
+ * reserve_per_cu per compute unit (and a few more).  Other kernels on the
+ * device (single runs, batches, another server's) get the workgroups of
+ * their launch spread over every compute unit, so with the default of none
+ * reserved they wait until the server has been idle for idle_ms and its
+ * kernel has left; with 2 per compute unit they run beside it.
  * ------------------------------------------------------------------------ */
 
 #define TMC_PENDING 1 /* positive status: the request has not finished */
@@ -288,11 +296,18 @@ typedef struct tmc_serve_options {
     uint32_t max_requests; /* resident requests (slots), <= 65536; 0 = 4096 */
     uint32_t block; /* threads per workgroup; 0 = 256 */
     uint64_t tally_words; /* device tally arena in 64-bit words; 0 = max_requests * 4 * max_shells */
-    uint64_t ring_chunks; /* chunk ring entries, power of two; 0 = 2^20 */
-    uint64_t chunk_work; /* target absorptions per chunk; 0 = 2^17 */
-    uint32_t max_photons_per_lane; /* cap on photons per lane in a chunk, <= 2^24; 0 = 16 */
+    uint64_t ring_chunks; /* chunk ring entries, power of two >= 2; 0 = 2^20 */
+    uint64_t chunk_work; /* target absorptions per chunk at light load; 0 = 2^17 */
+    uint32_t max_photons_per_lane; /* cap on photons per lane in a chunk at light load, <= 2^24; 0 = 16 */
     uint32_t idle_ms; /* the server stops the kernel after this long without requests; 0 = 50 */
     uint32_t math; /* TMC_MATH_* */
+    uint32_t max_chunk_scale; /* under load (unpublished work beyond two chunks per resident wave)
+                                 chunk_work and max_photons_per_lane grow with the backlog, up to
+                                 this factor (fewer wave drains per absorption); 1 = fixed chunks,
+                                 0 = 32, <= 1024 */
+    uint32_t reserve_per_cu; /* workgroup slots per compute unit the resident kernel leaves free
+                                for other work on the device; 0 = none (the whole device
+                                serves); 2 lets single runs and batches run beside the server */
     uint32_t reserved; /* 0 */
 } tmc_serve_options;
 
@@ -313,6 +328,9 @@ typedef struct tmc_serve_stats {
     uint64_t chunks_claimed; /* ring indices handed to waves (last observed) */
     uint64_t pending, active; /* requests waiting for admission / admitted, unfinished */
     double elapsed_s;
+    uint64_t cancelled; /* requests cancelled by tmc_serve_cancel or a bounded close */
+    uint64_t expired; /* requests cancelled by their deadline */
+    uint32_t grid, block; /* launch shape of the resident kernel */
 } tmc_serve_stats;
 
 typedef struct tmc_server tmc_server;
@@ -325,9 +343,23 @@ int tmc_serve_open(const tmc_serve_options* opt, tmc_server** server);
  * more shells than max_shells, TMC_E_NOMEM if its tally can never fit. */
 int tmc_serve_submit(tmc_server* server, const tmc_set* request, uint64_t* id);
 
+/* As tmc_serve_submit, with a deadline: a request not finished deadline_s
+ * seconds after its submission is cancelled (poll / wait then return
+ * TMC_E_DEADLINE).  deadline_s <= 0: none. */
+int tmc_serve_submit_ex(tmc_server* server, const tmc_set* request, double deadline_s, uint64_t* id);
+
+/* Cancels a request whose outcome has not been handed over: poll / wait
+ * then return TMC_E_CANCELLED (once).  A queued request is dropped; an
+ * admitted one publishes no further chunks, and its slot and arena region
+ * are reused once the chunks already published are done (every other
+ * request's result is unaffected); a finished one loses its result.
+ * TMC_E_INVALID for an unknown id. */
+int tmc_serve_cancel(tmc_server* server, uint64_t id);
+
 /* TMC_OK and the request's tally (4 * shells words, layout of
- * tiny_mc_gpu_run_ex) once it is done — the result is handed over once —,
- * TMC_PENDING before, TMC_E_INVALID for an unknown id. */
+ * tiny_mc_gpu_run_ex) once it is done — the outcome is handed over once —,
+ * TMC_PENDING before, TMC_E_CANCELLED / TMC_E_DEADLINE for a cancelled
+ * request, TMC_E_INVALID for an unknown id. */
 int tmc_serve_poll(tmc_server* server, uint64_t id, uint64_t* tally, tmc_request_stats* stats);
 
 /* As tmc_serve_poll, blocking up to timeout_s seconds (< 0: no limit). */
@@ -338,6 +370,14 @@ int tmc_serve_get_stats(tmc_server* server, tmc_serve_stats* stats);
 /* Finishes every submitted request, stops the kernel, frees the server.
  * Results not yet polled are discarded. */
 int tmc_serve_close(tmc_server* server);
+
+/* tmc_serve_close within a time limit: requests still unfinished timeout_s
+ * seconds after the call are cancelled; the chunks already claimed drain
+ * and the kernel leaves.  If that has not happened within another timeout_s
+ * (plus one second), returns TMC_E_TIMEOUT with the server still open (it
+ * accepts no new requests; call again to keep waiting).  timeout_s < 0: no
+ * limit (= tmc_serve_close). */
+int tmc_serve_close_timeout(tmc_server* server, double timeout_s);
 
 #ifdef __cplusplus
 }

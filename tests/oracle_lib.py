@@ -194,3 +194,42 @@ def tally_heat(tally, shells, mu_a, mu_s):
     w = u128(t[0:2 * S:2], t[1:2 * S:2])
     w2 = u128(t[2 * S::2], t[2 * S + 1::2])
     return oma * w * 2.0 ** -33, oma * oma * w2 * 2.0 ** -43
+
+
+# roulette survival: (r >> 8) < 0x199999, i.e. u = ((r >> 8) + 1) 2^-24 <= 0.1f
+P_SURVIVE = 1677721 / 2**24
+
+
+def cadence_moments(mu_a, mu_s, cycles=40):
+    """Exact mean and variance, per photon, of the absorptions, of the
+    absorbed energy sum (1 - a) w and of the sum of fixed-point heat2 images
+    RN(w w) 2^43, from the binary32 weight sequence: cycle 0 starts at
+    w = 1; each absorption deposits w, then w = RN(w a); once w < 0.001 the
+    photon survives with probability P_SURVIVE and the next cycle starts at
+    RN(w / 0.1f).  G survivals (P = p^G (1 - p)) walk cycles 0..G."""
+    f = np.float32
+    a32 = f(f(mu_s) / f(f(mu_s) + f(mu_a)))
+    oma = 1.0 - mu_s / (mu_s + mu_a)
+    w = f(1.0)
+    lens, energy, sq = [], [], []
+    for _ in range(cycles):
+        n, e, q = 0, 0.0, 0
+        while True:
+            n += 1
+            e += float(w)
+            q += int(float(f(w * w)) * 2.0**43)  # exact: w*w >= 2^-20 here
+            w = f(w * a32)
+            if w < f(0.001):
+                break
+        lens.append(n)
+        energy.append(oma * e)
+        sq.append(q)
+        w = f(w / f(0.1))
+    g = np.arange(cycles)
+    pg = P_SURVIVE**g * (1.0 - P_SURVIVE)
+    out = []
+    for per_cycle in (lens, energy, sq):
+        tot = np.cumsum(np.array(per_cycle, dtype=np.float64))
+        m = float((pg * tot).sum())
+        out.append((m, float((pg * tot * tot).sum()) - m * m))
+    return out

@@ -113,7 +113,9 @@ def test_direction_moments(orc, sampler):
 
 
 @pytest.mark.parametrize("sampler", [0, 1])
-@pytest.mark.parametrize("phys", [DEFAULT, dict(mu_a=10.0, mu_s=10.0, shells=51, microns_per_shell=100.0)])
+@pytest.mark.parametrize("phys", [DEFAULT, dict(mu_a=10.0, mu_s=10.0, shells=51, microns_per_shell=100.0),
+                                  dict(mu_a=0.5, mu_s=50.0, shells=201, microns_per_shell=20.0),
+                                  dict(mu_a=19.0, mu_s=1.0, shells=51, microns_per_shell=100.0)])
 def test_mirror32_agrees_with_walk64(orc, sampler, phys):
     """The FP32 mirror and the FP64 oracle walk the same uniforms: the same
     roulette decisions (identical absorption counts) and per-shell heat equal
@@ -169,3 +171,27 @@ def test_mirror_table_mode_equals_own_math(orc, sampler):
     assert not np.array_equal(a, c)
     d, _ = orc.mirror32(p, 77, 4096, sampler=sampler, tables=(-lg, sn, cs, sp * np.float32(1.001)))
     assert np.array_equal(a, d) == (sampler == 1)
+
+
+@pytest.mark.parametrize("phys,N", [(DEFAULT, 1 << 20), (dict(mu_a=10.0, mu_s=10.0, shells=51,
+                                                              microns_per_shell=100.0), 1 << 20),
+                                    (dict(mu_a=0.5, mu_s=50.0, shells=201, microns_per_shell=20.0), 1 << 16)])
+def test_cadence_moments_closed_form(orc, phys, N):
+    """The exact per-photon means of absorptions, absorbed energy and summed
+    heat2 images from the binary32 weight cadence (oracle_lib.cadence_moments,
+    used to check the full-size GPU run) agree with the FP32 mirror within
+    5 standard errors.  At default physics the band is narrow enough that a
+    cadence off by one absorption, or energy without the (1 - a) factor,
+    falls outside it."""
+    (mk, vk), (me, ve), (mq, vq) = oracle_lib.cadence_moments(phys["mu_a"], phys["mu_s"])
+    t, n = orc.mirror32(orc.params(**phys), 4321, N)
+    S = phys["shells"]
+    assert abs(n / N - mk) < 5 * math.sqrt(vk / N)
+    oma = 1.0 - phys["mu_s"] / (phys["mu_s"] + phys["mu_a"])
+    w = sum(int(a) + (int(b) << 64) for a, b in zip(t[0:2 * S:2], t[1:2 * S:2]))
+    assert abs(oma * w / 2.0**33 / N - me) < 5 * math.sqrt(ve / N)
+    q = sum(int(a) + (int(b) << 64) for a, b in zip(t[2 * S::2], t[2 * S + 1::2]))
+    assert abs(q / N - mq) < 5 * math.sqrt(vq / N)
+    if phys is DEFAULT:
+        assert abs((mk + 1) - n / N) > 5 * math.sqrt(vk / N)
+        assert abs(w / 2.0**33 / N - me) > 5 * math.sqrt(ve / N)

@@ -8,6 +8,9 @@ tiny_mc.c:43-49 becomes one ``run`` call.
 """
 from ._lib import (  # noqa: F401
     PENDING,
+    E_CANCELLED,
+    E_DEADLINE,
+    E_TIMEOUT,
     BatchOptions,
     RequestStats,
     Server,

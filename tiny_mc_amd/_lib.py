@@ -34,7 +34,13 @@ _STATUS = {
     -4: "out of memory",
     -5: "photon index range exceeds 2^63",
     -6: "unsupported option",
+    -8: "request cancelled",
+    -9: "request deadline passed",
+    -10: "server close timed out",
 }
+E_CANCELLED = -8
+E_DEADLINE = -9
+E_TIMEOUT = -10
 
 
 class TmcError(RuntimeError):
@@ -176,19 +182,24 @@ def run(params: Params, seed: int):
     return heat, heat2
 
 
-def run_raw(params: Params, seed: int, *, sampler=SAMPLER_ANALYTIC, first_photon=0, max_interactions=0,
-            block=0, grid=0, chunk=0, precision=PRECISION_FP32, math=MATH_HARDWARE):
-    """Integer tallies (uint64 array of 4*shells: 128-bit (lo, hi) sums) and RunStats."""
+def _run_raw(entry, params, seed, sampler, first_photon, max_interactions, block, grid, chunk, precision, math):
     lib = load()
     tally = np.zeros(4 * params.shells, dtype=np.uint64)
     opt = _Options(sampler, block, grid, chunk, first_photon, max_interactions, precision, math)
     st = _Stats()
     c = params._c()
-    _check(lib.tiny_mc_gpu_run_ex(ctypes.byref(c), seed, ctypes.byref(opt), _ptr(tally, ctypes.c_uint64),
-                                  ctypes.byref(st)), "tiny_mc_gpu_run_ex")
+    _check(getattr(lib, entry)(ctypes.byref(c), seed, ctypes.byref(opt), _ptr(tally, ctypes.c_uint64),
+                               ctypes.byref(st)), entry)
     stats = RunStats(st.photons, st.interactions, st.truncated, st.kernel_ms, st.total_ms, st.grid, st.block,
                      st.chunk, st.sampler, st.precision, st.math)
     return tally, stats
+
+
+def run_raw(params: Params, seed: int, *, sampler=SAMPLER_ANALYTIC, first_photon=0, max_interactions=0,
+            block=0, grid=0, chunk=0, precision=PRECISION_FP32, math=MATH_HARDWARE):
+    """Integer tallies (uint64 array of 4*shells: 128-bit (lo, hi) sums) and RunStats."""
+    return _run_raw("tiny_mc_gpu_run_ex", params, seed, sampler, first_photon, max_interactions, block, grid, chunk,
+                    precision, math)
 
 
 def tally_to_heat(params: Params, tally):
@@ -532,6 +543,8 @@ class _ServeOptions(ctypes.Structure):
         ("max_photons_per_lane", ctypes.c_uint32),
         ("idle_ms", ctypes.c_uint32),
         ("math", ctypes.c_uint32),
+        ("max_chunk_scale", ctypes.c_uint32),
+        ("reserve_per_cu", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
     ]
 
@@ -560,6 +573,10 @@ class _ServeStats(ctypes.Structure):
         ("pending", ctypes.c_uint64),
         ("active", ctypes.c_uint64),
         ("elapsed_s", ctypes.c_double),
+        ("cancelled", ctypes.c_uint64),
+        ("expired", ctypes.c_uint64),
+        ("grid", ctypes.c_uint32),
+        ("block", ctypes.c_uint32),
     ]
 
 
@@ -579,53 +596,71 @@ def _bind_serve(lib):
     P = ctypes.POINTER
     lib.tmc_serve_open.argtypes = [P(_ServeOptions), P(ctypes.c_void_p)]
     lib.tmc_serve_submit.argtypes = [ctypes.c_void_p, P(_Set), P(ctypes.c_uint64)]
+    lib.tmc_serve_submit_ex.argtypes = [ctypes.c_void_p, P(_Set), ctypes.c_double, P(ctypes.c_uint64)]
+    lib.tmc_serve_cancel.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    lib.tmc_serve_close_timeout.argtypes = [ctypes.c_void_p, ctypes.c_double]
     lib.tmc_serve_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, P(ctypes.c_uint64), P(_RequestStats)]
     lib.tmc_serve_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, P(ctypes.c_uint64),
                                    P(_RequestStats)]
     lib.tmc_serve_get_stats.argtypes = [ctypes.c_void_p, P(_ServeStats)]
     lib.tmc_serve_close.argtypes = [ctypes.c_void_p]
-    for n in ("tmc_serve_open", "tmc_serve_submit", "tmc_serve_poll", "tmc_serve_wait", "tmc_serve_get_stats",
-              "tmc_serve_close"):
+    for n in ("tmc_serve_open", "tmc_serve_submit", "tmc_serve_submit_ex", "tmc_serve_cancel", "tmc_serve_poll",
+              "tmc_serve_wait", "tmc_serve_get_stats", "tmc_serve_close", "tmc_serve_close_timeout"):
         getattr(lib, n).restype = ctypes.c_int
     lib._tmc_serve_bound = True
 
 
 class Server:
-    """Request loop over a resident persistent kernel (tmc_serve_*)."""
+    """Request loop over a resident persistent kernel (tmc_serve_*).
+
+    Used as a context manager it closes with a time limit (``close_timeout``
+    seconds for unfinished requests, then cancelled ones drain), so a
+    request that never finishes fails the caller instead of holding it."""
 
     def __init__(self, sampler=SAMPLER_ANALYTIC, max_shells=0, max_requests=0, block=0, tally_words=0,
-                 ring_chunks=0, chunk_work=0, max_photons_per_lane=0, idle_ms=0, math=MATH_HARDWARE):
+                 ring_chunks=0, chunk_work=0, max_photons_per_lane=0, idle_ms=0, math=MATH_HARDWARE,
+                 max_chunk_scale=0, reserve_per_cu=0, close_timeout=120.0):
         lib = load()
         _bind_serve(lib)
         self._lib = lib
         o = _ServeOptions(sampler, max_shells, max_requests, block, tally_words, ring_chunks, chunk_work,
-                          max_photons_per_lane, idle_ms, math, 0)
+                          max_photons_per_lane, idle_ms, math, max_chunk_scale, reserve_per_cu, 0)
         h = ctypes.c_void_p()
         _check(lib.tmc_serve_open(ctypes.byref(o), ctypes.byref(h)), "tmc_serve_open")
         self._h = h
         self._shells = {}
+        self.close_timeout = close_timeout
 
-    def submit(self, params: Params, seed: int, first_photon: int = 0):
+    def submit(self, params: Params, seed: int, first_photon: int = 0, deadline=None):
+        """Queues a request (deadline: seconds after submission, or None)."""
         s = _Set(params._c(), seed, first_photon)
         rid = ctypes.c_uint64()
-        _check(self._lib.tmc_serve_submit(self._h, ctypes.byref(s), ctypes.byref(rid)), "tmc_serve_submit")
+        _check(self._lib.tmc_serve_submit_ex(self._h, ctypes.byref(s), float(deadline or 0.0), ctypes.byref(rid)),
+               "tmc_serve_submit_ex")
         self._shells[rid.value] = params.shells
         return rid.value
+
+    def cancel(self, rid):
+        _check(self._lib.tmc_serve_cancel(self._h, rid), "tmc_serve_cancel")
 
     def _result(self, rc, rid, t, st, what):
         if rc == PENDING:
             return None
+        if rc in (0, E_CANCELLED, E_DEADLINE):  # the outcome is handed over once
+            self._shells.pop(rid, None)
         _check(rc, what)
-        self._shells.pop(rid, None)
         return t, RequestStats(st.photons, st.interactions, st.submit_s, st.admit_s, st.done_s, st.latency_ms)
 
     def poll(self, rid):
+        """(tally, RequestStats) once done, None while pending; TmcError with
+        status E_CANCELLED / E_DEADLINE for a cancelled request."""
         t = np.zeros(4 * self._shells[rid], dtype=np.uint64)
         st = _RequestStats()
         rc = self._lib.tmc_serve_poll(self._h, rid, _ptr(t, ctypes.c_uint64), ctypes.byref(st))
         return self._result(rc, rid, t, st, "tmc_serve_poll")
 
     def wait(self, rid, timeout=-1.0):
+        """As poll, blocking up to `timeout` seconds (< 0: no limit)."""
         t = np.zeros(4 * self._shells[rid], dtype=np.uint64)
         st = _RequestStats()
         rc = self._lib.tmc_serve_wait(self._h, rid, timeout, _ptr(t, ctypes.c_uint64), ctypes.byref(st))
@@ -636,13 +671,20 @@ class Server:
         _check(self._lib.tmc_serve_get_stats(self._h, ctypes.byref(st)), "tmc_serve_get_stats")
         return {f: getattr(st, f) for f, _ in _ServeStats._fields_}
 
-    def close(self):
+    def close(self, timeout=None):
+        """Finishes the submitted requests and frees the server; with a
+        timeout, cancels what is unfinished after `timeout` seconds and
+        raises TmcError(E_TIMEOUT) if the kernel has not left within as long
+        again (the server then stays open; close may be called again)."""
         if self._h:
-            h, self._h = self._h, None
-            _check(self._lib.tmc_serve_close(h), "tmc_serve_close")
+            rc = self._lib.tmc_serve_close_timeout(self._h, -1.0 if timeout is None else float(timeout))
+            if rc == E_TIMEOUT:
+                _check(rc, "tmc_serve_close_timeout")
+            self._h = None
+            _check(rc, "tmc_serve_close_timeout")
 
     def __enter__(self):
         return self
 
     def __exit__(self, *exc):
-        self.close()
+        self.close(self.close_timeout)

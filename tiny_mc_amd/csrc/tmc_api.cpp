@@ -13,7 +13,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
+#include <vector>
 
 #include "../../include/tiny_mc_gpu.h"
 #include "tmc_context.h"
@@ -33,10 +35,8 @@ int destroy_context(Context* c)
     if (!c)
         return TMC_OK;
     (void)hipSetDevice(c->device);
-    if (c->d_tally)
-        (void)hipFree(c->d_tally);
-    if (c->d_counters)
-        (void)hipFree(c->d_counters);
+    if (c->d_work)
+        tmc::release_device(c->d_work);
     if (c->ev_start)
         (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop)
@@ -72,10 +72,10 @@ int create_context(int device, Context** out)
     }
     c->compute_units = prop.multiProcessorCount;
     std::snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
-    /* default-priority stream (never a prioritised one) */
+    /* default-priority stream (never a prioritised one); no device memory
+     * yet: the single-run words are allocated at the first run */
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess
-        || hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess
-        || hipMalloc((void**)&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess) {
+        || hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
         destroy_context(c);
         return TMC_E_HIP;
     }
@@ -102,21 +102,62 @@ int tmc::acquire_context(Context** out)
 
 namespace {
 
-int ensure_tally(Context* c, size_t words)
+/* The single-run words at the largest shell count, once per context. */
+int ensure_work(Context* c)
 {
-    if (c->tally_words >= words)
+    if (c->d_work)
         return TMC_OK;
-    if (c->d_tally)
-        (void)hipFree(c->d_tally);
-    c->d_tally = nullptr;
-    c->tally_words = 0;
-    if (hipMalloc((void**)&c->d_tally, words * sizeof(unsigned long long)) != hipSuccess)
+    if (hipMalloc((void**)&c->d_work, tmc::run_work_words(TMC_MAX_SHELLS) * sizeof(unsigned long long)) != hipSuccess) {
+        c->d_work = nullptr;
         return TMC_E_NOMEM;
-    c->tally_words = words;
+    }
     return TMC_OK;
 }
 
+/* Releases made while a serving kernel is resident (tmc_context.h). */
+std::mutex g_release_mutex;
+int g_serving_kernels = 0;
+std::vector<std::function<void()>> g_deferred;
+
 } // namespace
+
+void tmc::release(std::function<void()> fn)
+{
+    std::lock_guard<std::mutex> lock(g_release_mutex);
+    if (g_serving_kernels > 0)
+        g_deferred.push_back(std::move(fn));
+    else
+        fn();
+}
+
+void tmc::release_device(void* p)
+{
+    if (p)
+        release([p]() { (void)hipFree(p); });
+}
+
+void tmc::release_host(void* p)
+{
+    if (p)
+        release([p]() { (void)hipHostFree(p); });
+}
+
+void tmc::serving_kernel_started()
+{
+    std::lock_guard<std::mutex> lock(g_release_mutex);
+    ++g_serving_kernels;
+}
+
+void tmc::serving_kernel_left()
+{
+    std::lock_guard<std::mutex> lock(g_release_mutex);
+    if (--g_serving_kernels > 0)
+        return;
+    std::vector<std::function<void()>> todo;
+    todo.swap(g_deferred);
+    for (auto& fn : todo)
+        fn();
+}
 
 /* Parameter checks.  The reference has none (params.h:14 only warns that
  * MU_A must be non-zero: with mu_a = 0 the weight never reaches the cutoff
@@ -204,8 +245,10 @@ double u128_to_double(uint64_t lo, uint64_t hi)
     return std::ldexp((double)hi, 64) + (double)lo;
 }
 
-int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options* o, uint64_t* tally,
-               tmc_run_stats* st)
+} // namespace
+
+int tmc::run_walk(Context* c, const tmc_params* p, uint64_t seed, const tmc_options* o, unsigned long long* work,
+                  uint64_t* tally, tmc_run_stats* st)
 {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t S = p->shells;
@@ -251,10 +294,8 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
         chunk = 64u * (uint32_t)per_lane;
     }
 
-    int rc = ensure_tally(c, 4 * (size_t)S);
-    if (rc != TMC_OK)
-        return rc;
-
+    unsigned long long* const d_counters = work;
+    unsigned long long* const d_tally = work + tmc::kRunCounterWords;
     tmc::WalkArgs a;
     std::memset(&a, 0, sizeof a);
     a.seed = seed;
@@ -281,12 +322,13 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
      * image of w = 1) accumulate: keep that below 2^63. */
     a.flush_interval = (1u << 14) / waves_per_wg;
     a.kcap = (uint32_t)kcap;
-    a.counters = c->d_counters;
-    a.tally = c->d_tally;
+    a.counters = d_counters;
+    a.tally = d_tally;
 
     float kernel_ms = 0.0f;
-    if (hipMemsetAsync(c->d_tally, 0, 4 * (size_t)S * sizeof(unsigned long long), c->stream) != hipSuccess
-        || hipMemsetAsync(c->d_counters, 0, 4 * sizeof(unsigned long long), c->stream) != hipSuccess)
+    /* a fill kernel, not a memset: the same sequence serves callers that
+     * cannot have memset nodes in their streams */
+    if (tmc::launch_clear(work, tmc::run_work_words(S), c->stream) != hipSuccess)
         return TMC_E_HIP;
     unsigned long long counters[4] = { 0, 0, 0, 0 };
     if (p->photons > 0) {
@@ -297,9 +339,9 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
         if (hipEventRecord(c->ev_stop, c->stream) != hipSuccess)
             return TMC_E_HIP;
     }
-    if (hipMemcpyAsync(tally, c->d_tally, 4 * (size_t)S * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream)
+    if (hipMemcpyAsync(tally, d_tally, 4 * (size_t)S * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream)
             != hipSuccess
-        || hipMemcpyAsync(counters, c->d_counters, sizeof counters, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        || hipMemcpyAsync(counters, d_counters, sizeof counters, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
         return TMC_E_HIP;
     if (hipStreamSynchronize(c->stream) != hipSuccess)
         return TMC_E_HIP;
@@ -320,8 +362,6 @@ int run_locked(Context* c, const tmc_params* p, uint64_t seed, const tmc_options
     }
     return TMC_OK;
 }
-
-} // namespace
 
 TMC_EXPORT void tmc_options_default(tmc_options* opt)
 {
@@ -349,7 +389,10 @@ TMC_EXPORT int tiny_mc_gpu_run_ex(const tmc_params* p, uint64_t seed, const tmc_
     rc = tmc::acquire_context(&c);
     if (rc != TMC_OK)
         return rc;
-    return run_locked(c, p, seed, &o, tally, stats);
+    rc = ensure_work(c);
+    if (rc != TMC_OK)
+        return rc;
+    return tmc::run_walk(c, p, seed, &o, c->d_work, tally, stats);
 }
 
 TMC_EXPORT int tiny_mc_gpu_tally_to_heat(const tmc_params* p, const uint64_t* tally, double* heat, double* heat2)
@@ -452,8 +495,7 @@ struct DevBufs {
     ~DevBufs()
     {
         for (void* q : p)
-            if (q)
-                (void)hipFree(q);
+            tmc::release_device(q);
     }
 };
 

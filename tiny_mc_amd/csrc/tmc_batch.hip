@@ -184,15 +184,19 @@ enum ClaimResult { CLAIM_GOT, CLAIM_EMPTY, CLAIM_DONE };
 /* Tries to give the wave its next chunk (wave-uniform result).
  * Batch: the next index of the fixed queue, or DONE past its end.
  * Serving: the wave takes a ticket (the next ring index) once and keeps it
- * until the tail passes it; meanwhile the call returns EMPTY without
+ * until its entry is published; meanwhile the call returns EMPTY without
  * waiting, so the wave's live lanes keep walking.  The tail is read from
- * the device cache; when the ticket lies past it and the cache is older
- * than the refresh interval, the one wave that wins the refresh reads the
- * host's words (stop first: a stop seen means the tail read after it is
- * final).  DONE once this launch is asked to stop and the ticket lies past
- * the tail. */
+ * the wave's copy of the control words (`group`); when the ticket lies past
+ * it and the last refresh is older than the refresh interval, the one wave
+ * of group 0 that wins the refresh reads the host's words (stop first, with
+ * acquire: a stop seen means the tail read after it is final) and its lanes
+ * write them into every copy.  Below the tail the entry is taken once its
+ * sequence number equals the ticket (RingEntry).  DONE once this launch is
+ * asked to stop and the ticket lies past the tail: the host asks only when
+ * every published entry was taken, so an entry at the ticket can only come
+ * after the launch has ended. */
 template <bool SERVE>
-__device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t lane, uint64_t& ticket,
+__device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t lane, uint32_t group, uint64_t& ticket,
                                                  BatchChunk& ch)
 {
     if (!SERVE) {
@@ -205,40 +209,59 @@ __device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t la
         ch = a.chunks[c];
         return CLAIM_GOT;
     }
-    uint64_t t = ticket, w0 = 0, w1 = 0;
-    uint32_t res = CLAIM_EMPTY;
+    ServeCache* const copy = a.dctl + group;
+    uint64_t t = ticket, tail = 0, w0 = 0, w1 = 0;
+    uint32_t res = CLAIM_EMPTY, refresh = 0;
     if (lane == 0) {
         if (t == ~0ull)
             t = a.ticket_base + atomicAdd(a.counters, 1ull);
-        uint64_t tail = load_sys(&a.dctl->tail);
-        uint64_t stop = 0;
-        bool stop_read = false;
-        if (t >= tail) {
+        /* the cached tail only tells when to look: the entry's own sequence
+         * number decides, so relaxed loads suffice here */
+        tail = load_sys(&copy->tail);
+        if (t >= tail && group == 0) {
+            uint64_t* const stamp = &a.dctl[TMC_SERVE_CACHE_COPIES].tail;
             const uint64_t now = __builtin_amdgcn_s_memrealtime();
-            uint64_t last = load_sys(&a.dctl->time);
-            if (now - last >= a.refresh_ticks
-                && __hip_atomic_compare_exchange_strong(&a.dctl->time, &last, now, __ATOMIC_RELAXED,
-                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
-                stop = load_sys(&a.hctl->stop);
-                wait_vmem();
-                const uint64_t ht = load_sys(&a.hctl->tail);
-                __hip_atomic_fetch_max(&a.dctl->tail, ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                __hip_atomic_exchange(&a.dctl->stop, stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                tail = ht > tail ? ht : tail;
-                stop_read = true;
-            }
+            uint64_t last = load_sys(stamp);
+            refresh = now - last >= a.refresh_ticks
+                && __hip_atomic_compare_exchange_strong(stamp, &last, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_SYSTEM);
         }
+    }
+    uint64_t stop = 0;
+    const bool refreshed = rfl32(refresh) != 0;
+    if (refreshed) {
+        uint64_t ht = 0;
+        if (lane == 0) {
+            stop = __hip_atomic_load(&a.hctl->stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            ht = load_sys(&a.hctl->tail);
+        }
+        stop = rfl64(stop);
+        ht = rfl64(ht);
+        if (lane < TMC_SERVE_CACHE_COPIES) {
+            __hip_atomic_fetch_max(&a.dctl[lane].tail, ht, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_exchange(&a.dctl[lane].stop, stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        tail = ht > tail ? ht : tail;
+    }
+    if (lane == 0) {
         if (t < tail) {
-            const uint64_t* slot = (const uint64_t*)(a.chunks + (t & a.ring_mask));
-            w0 = load_sys(slot);
-            w1 = load_sys(slot + 1);
-            res = CLAIM_GOT;
-            /* progress report for the host's backlog bound (a stale
-             * value only makes the host publish less) */
-            store_sys(&a.hctl->claimed, t + 1);
+            RingEntry* e = a.ring + (t & a.ring_mask);
+            /* the host stores seq last, with release: seq == t (acquire)
+             * means the entry's fields are the ones published for t */
+            if (__hip_atomic_load(&e->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) == t) {
+                w0 = load_sys((const uint64_t*)e);
+                w1 = load_sys(&e->first);
+                /* consumed (release: after the field reads): the host may
+                 * now reuse the slot */
+                __hip_atomic_store(&e->seq, t | TMC_RING_CONSUMED, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                res = CLAIM_GOT;
+                /* progress report for the host's backlog estimate (a stale
+                 * value only makes the host publish less) */
+                store_sys(&a.hctl->claimed, t + 1);
+            }
         } else {
-            if (!stop_read)
-                stop = load_sys(&a.dctl->stop);
+            if (!refreshed)
+                stop = load_sys(&copy->stop);
             if (stop == a.launch_id)
                 res = CLAIM_DONE;
         }
@@ -273,9 +296,28 @@ __device__ __forceinline__ void finish_request(const BatchArgs& a, const SetStat
         take_sys(st.done);
         store_sys((uint64_t*)a.flags + 2 * (uint64_t)st.slot + 1, absorbed);
     }
+    /* every lane's mirror stores before lane 0's flag (the release below
+     * orders only lane 0's own) */
     wait_vmem();
+    __builtin_amdgcn_wave_barrier();
     if (lane == 0)
-        store_sys((uint64_t*)a.flags + 2 * (uint64_t)st.slot, st.token);
+        __hip_atomic_store((uint64_t*)a.flags + 2 * (uint64_t)st.slot, st.token, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+/* Serving: counts `photons` of the request as completed without walking
+ * them (a cancellation's photons that were never published); the wave whose
+ * count completes the request finishes it. */
+__device__ __forceinline__ void publish_skip(const BatchArgs& a, const SetState& st, uint32_t lane, uint64_t photons)
+{
+    uint32_t last = 0;
+    if (lane == 0) {
+        const unsigned long long old = __hip_atomic_fetch_add(st.done, (unsigned long long)photons, __ATOMIC_ACQ_REL,
+                                                              __HIP_MEMORY_SCOPE_AGENT);
+        last = old + photons == st.photons;
+    }
+    if (rfl32(last))
+        finish_request(a, st, lane);
 }
 
 /* Flushes the slot into the set's sums, then publishes the photons completed
@@ -304,14 +346,19 @@ __device__ __forceinline__ void flush_and_publish(unsigned long long* slot, cons
     }
     steps = 0;
     if (SERVE) {
+        /* every lane's sum updates before lane 0's count (the release below
+         * orders only lane 0's own) */
         wait_vmem();
+        __builtin_amdgcn_wave_barrier();
         uint32_t last = 0;
         if (lane == 0) {
             __hip_atomic_fetch_add(st.done + 1, (unsigned long long)absorbed, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
-            wait_vmem();
+            /* release: the sums and absorptions before the count; acquire:
+             * the wave whose count completes the request sees every other
+             * wave's sums (each counted after its own) */
             const unsigned long long old = __hip_atomic_fetch_add(st.done, (unsigned long long)completed,
-                                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                                                                  __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             last = completed && old + completed == st.photons;
         }
         if (rfl32(last))
@@ -352,7 +399,10 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
      * whose photon ended cost scalar instructions on every step) */
     uint32_t absorbed = 0, steps = 0, launched = 0, live_pub = 0;
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
-    uint32_t poll_wait = 0; /* serving: steps before re-polling an unpublished ticket */
+    /* serving: steps before re-polling an unpublished ticket (low 16 bits)
+     * and the number of failed polls in a row (high bits), one scalar word */
+    uint32_t poll = 0;
+    const uint32_t group = SERVE ? (blockIdx.x * (blockDim.x >> 6) + wave) % TMC_SERVE_CACHE_COPIES : 0u;
 
     /* a lane holds a live photon while its weight is >= 0; -1 marks a lane
      * without one (a comparison, so each ballot of it is one vector compare) */
@@ -370,15 +420,20 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             /* the pool is empty: claim a chunk, or switch sets once drained
              * (PENDING is set by a claim with the pool empty and stays so
              * until the switch) */
-            if (SERVE && poll_wait && dead != ~0ull) {
-                --poll_wait; /* live lanes keep walking; poll again later */
+            if (SERVE && (poll & 0xFFFFu) && dead != ~0ull) {
+                --poll; /* live lanes keep walking; poll again later */
             } else if (!(flags & (PENDING | QUEUE_DONE))) {
                 BatchChunk ch;
-                const ClaimResult r = try_claim<SERVE>(a, lane, ticket, ch);
+                const ClaimResult r = try_claim<SERVE>(a, lane, group, ticket, ch);
                 if (r == CLAIM_GOT) {
+                    poll = 0;
                     if (ch.set == cur_set) {
-                        pool_next = ch.first;
-                        pool_end = ch.first + ch.count;
+                        if (SERVE && ch.count == 0) {
+                            publish_skip(a, st, lane, ch.first); /* cancelled photons */
+                        } else {
+                            pool_next = ch.first;
+                            pool_end = ch.first + ch.count;
+                        }
                     } else {
                         pend = ch;
                         flags |= PENDING;
@@ -386,7 +441,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 } else if (r == CLAIM_DONE) {
                     flags |= QUEUE_DONE;
                 } else if (SERVE && dead != ~0ull) {
-                    poll_wait = 16;
+                    poll = (poll & 0xFFFF0000u) | 16u;
                 } else if (SERVE) {
                     /* ran dry: publish what was completed, then wait a
                      * little.  A wave never leaves on its own: the ticket it
@@ -399,7 +454,13 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                         flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
                         launched = live_pub = 0;
                     }
-                    __builtin_amdgcn_s_sleep(64);
+                    /* back off while dry: 1.7 us after the first failed
+                     * poll, doubling to 27 us (a new chunk waits at most
+                     * that long for an idle wave) */
+                    const uint32_t fails = poll >> 16;
+                    for (uint32_t k = 0; k < (1u << (fails < 4u ? fails : 4u)); ++k)
+                        __builtin_amdgcn_s_sleep(64);
+                    poll = (fails < 4u ? fails + 1u : 4u) << 16;
                 }
             }
             if ((flags & PENDING) && dead == ~0ull) {
@@ -413,8 +474,13 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 load_set<SAMPLER, SERVE>(st, a, cur_set & slot_mask, lane);
                 /* every lane is empty here: only a small cap needs checks */
                 flags = st.kcap <= TMC_SLOT_FLUSH_STEPS ? (flags | CAPPING) : (flags & ~CAPPING);
-                pool_next = pend.first;
-                pool_end = pend.first + pend.count;
+                if (SERVE && pend.count == 0) {
+                    publish_skip(a, st, lane, pend.first); /* cancelled photons */
+                    pool_next = pool_end = 0;
+                } else {
+                    pool_next = pend.first;
+                    pool_end = pend.first + pend.count;
+                }
                 flags &= ~PENDING;
             }
         }

@@ -75,28 +75,27 @@ struct tmc_batch_plan {
 
 namespace {
 
+/* Device and pinned memory, and the graph, go through tmc::release, which
+ * defers them while a serving kernel is resident (hipFree and hipHostFree
+ * would wait for it). */
 void plan_free(tmc_batch_plan* p)
 {
     if (!p)
         return;
-    if (p->exec)
-        (void)hipGraphExecDestroy(p->exec);
-    if (p->graph)
-        (void)hipGraphDestroy(p->graph);
-    if (p->d_sets)
-        (void)hipFree(p->d_sets);
-    if (p->d_chunks)
-        (void)hipFree(p->d_chunks);
-    if (p->d_tally)
-        (void)hipFree(p->d_tally);
-    if (p->d_done)
-        (void)hipFree(p->d_done);
-    if (p->d_counters)
-        (void)hipFree(p->d_counters);
-    if (p->h_tally)
-        (void)hipHostFree(p->h_tally);
-    if (p->h_counters)
-        (void)hipHostFree(p->h_counters);
+    if (p->exec || p->graph) {
+        hipGraphExec_t exec = p->exec;
+        hipGraph_t graph = p->graph;
+        tmc::release([exec, graph]() {
+            if (exec)
+                (void)hipGraphExecDestroy(exec);
+            if (graph)
+                (void)hipGraphDestroy(graph);
+        });
+    }
+    for (void* d : { (void*)p->d_sets, (void*)p->d_chunks, (void*)p->d_tally, (void*)p->d_done, (void*)p->d_counters })
+        tmc::release_device(d);
+    tmc::release_host(p->h_tally);
+    tmc::release_host(p->h_counters);
     delete p;
 }
 
@@ -349,10 +348,14 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
         plan_free(p);
         return TMC_E_NOMEM;
     }
-    if ((nsets && hipMemcpy(p->d_sets, hs.data(), nsets * sizeof(tmc::BatchSet), hipMemcpyHostToDevice) != hipSuccess)
+    if ((nsets
+         && hipMemcpyAsync(p->d_sets, hs.data(), nsets * sizeof(tmc::BatchSet), hipMemcpyHostToDevice, c->stream)
+             != hipSuccess)
         || (!hc.empty()
-            && hipMemcpy(p->d_chunks, hc.data(), hc.size() * sizeof(tmc::BatchChunk), hipMemcpyHostToDevice)
-                != hipSuccess)) {
+            && hipMemcpyAsync(p->d_chunks, hc.data(), hc.size() * sizeof(tmc::BatchChunk), hipMemcpyHostToDevice,
+                              c->stream)
+                != hipSuccess)
+        || hipStreamSynchronize(c->stream) != hipSuccess) {
         plan_free(p);
         return TMC_E_HIP;
     }

@@ -32,6 +32,28 @@ struct BatchChunk {
 };
 static_assert(sizeof(BatchChunk) == 16, "BatchChunk layout");
 
+/* Serving: one entry of the chunk ring (32 bytes, host-coherent memory).
+ * The host writes set, count and first, then stores seq = the entry's ring
+ * index t with release; a wave holding ticket t takes the entry only when
+ * it reads seq == t (acquire), and after reading it stores seq = t |
+ * TMC_RING_CONSUMED (release).  The host rewrites the slot for index t +
+ * ring only once it reads that mark, so no entry is overwritten before its
+ * wave has read it, whatever the timing.
+ * count == 0 marks a cancellation: `first` photons of request `set` that
+ * will never be published; the wave adds them to the request's completed
+ * photons, so the request finishes (and frees its slot) once the chunks
+ * already published are done. */
+struct RingEntry {
+    uint32_t set; /* slot | generation << TMC_SERVE_SLOT_BITS */
+    uint32_t count; /* photons, or 0: cancellation of `first` photons */
+    uint64_t first; /* offset within the request, or the cancelled photons */
+    uint64_t seq; /* ring index of the entry, | TMC_RING_CONSUMED once read */
+    uint64_t pad;
+};
+static_assert(sizeof(RingEntry) == 32, "RingEntry layout");
+
+#define TMC_RING_CONSUMED (1ull << 63)
+
 /* Serving: control words the host writes in host-coherent memory. */
 struct ServeControl {
     uint64_t tail; /* chunks published so far (ring indices below it are valid) */
@@ -40,19 +62,26 @@ struct ServeControl {
     uint64_t pad[5];
 };
 
-/* Serving: device-memory copy of the control words, refreshed from the host
- * by one wave at a time, at most once per refresh interval, so the polling
- * of thousands of waves stays off the host bus. */
+/* Serving: device-memory copies of the control words, each on its own
+ * 64-byte line, refreshed from the host by one wave at a time, at most once
+ * per refresh interval, so the polling of thousands of waves stays off the
+ * host bus.  Wave w polls copy w % TMC_SERVE_CACHE_COPIES: with a single
+ * line, thousands of waves without work polled it every few microseconds
+ * and the waves still walking waited behind that traffic on every poll.
+ * The line after the copies holds the time of the last refresh (its tail
+ * word), read only by the waves of copy 0, which elect the refresher. */
 struct ServeCache {
     uint64_t tail;
     uint64_t stop;
-    uint64_t time; /* realtime of the last refresh */
-    uint64_t pad[5];
+    uint64_t pad[6];
 };
+static_assert(sizeof(ServeCache) == 64, "ServeCache layout");
+#define TMC_SERVE_CACHE_COPIES 64u
 
 struct BatchArgs {
     const BatchSet* sets; /* batch: device table; serve: host-coherent table */
-    const BatchChunk* chunks; /* batch: device queue; serve: host-coherent ring */
+    const BatchChunk* chunks; /* batch: device queue */
+    RingEntry* ring; /* serve: host-coherent chunk ring */
     uint32_t nchunks; /* batch: queue length of this pass */
     uint32_t cursor; /* batch: index of this pass's chunk cursor in counters */
     uint32_t shells_max; /* LDS slot size in shells */
@@ -65,7 +94,7 @@ struct BatchArgs {
     unsigned long long* done; /* per set: [2s] photons completed and flushed, [2s+1] absorptions */
     /* serving only */
     ServeControl* hctl; /* host-coherent control words */
-    ServeCache* dctl; /* device cache of the control words */
+    ServeCache* dctl; /* device copies of the control words (TMC_SERVE_CACHE_COPIES + 1 lines) */
     unsigned long long* out; /* host-coherent mirror of the tally pool: sums of finished requests */
     unsigned long long* flags; /* host-coherent, per slot: [2s] token of the finished request, [2s+1]
                                   its absorptions */

@@ -42,6 +42,10 @@ hipError_t launch_walk(uint32_t sampler, uint32_t precision, uint32_t math, cons
 hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t math, uint32_t block, uint32_t shells,
                           int* wg_per_cu);
 
+/* Zeroes n 64-bit words at p on `stream` (a kernel, so the sequence holds no
+ * memset node). */
+hipError_t launch_clear(unsigned long long* p, size_t n, hipStream_t stream);
+
 hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out, hipStream_t stream);
 hipError_t launch_check_shell(float spm, float shell_max, uint32_t first, uint64_t count, unsigned long long* out,
                               hipStream_t stream);

@@ -247,6 +247,22 @@ hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t math, u
                                                         lds);
 }
 
+__global__ void clear_kernel(unsigned long long* p, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = 0;
+}
+
+hipError_t launch_clear(unsigned long long* p, size_t n, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    const size_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(clear_kernel, dim3(blocks < 1024 ? (uint32_t)blocks : 1024u), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
 __global__ void philox_kernel(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -44,6 +44,7 @@ typedef enum tmc_status {
     TMC_E_NOMEM = -4, /* device or host allocation failed */
     TMC_E_RANGE = -5, /* photon index range first_photon + photons exceeds 2^63 */
     TMC_E_UNSUPPORTED = -6, /* option or library build not available */
+    TMC_E_TOO_LARGE = -7, /* more shells than the static scratch of tiny_mc_gpu_run_noalloc holds */
     TMC_E_CANCELLED = -8, /* serving: the request was cancelled (tmc_serve_cancel, tmc_serve_close_timeout) */
     TMC_E_DEADLINE = -9, /* serving: the request's deadline passed before it finished */
     TMC_E_TIMEOUT = -10 /* serving: tmc_serve_close_timeout ran out of time; the server is still open */
@@ -125,6 +126,26 @@ int tiny_mc_gpu_run(const tmc_params* p, uint64_t seed, double* heat, double* he
  * integers.  stats may be NULL. */
 int tiny_mc_gpu_run_ex(const tmc_params* p, uint64_t seed, const tmc_options* opt, uint64_t* tally,
                        tmc_run_stats* stats);
+
+/* Largest shell count of tiny_mc_gpu_run_noalloc: its static device scratch
+ * holds 4 + 4 * TMC_NOALLOC_MAX_SHELLS 64-bit words (128 KiB) per device. */
+#define TMC_NOALLOC_MAX_SHELLS 4096
+
+/* tiny_mc_gpu_run_ex without any device or host allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code that runs
+ * while a HIP graph is captured elsewhere in the process, tight serving
+ * loops, hosts that forbid allocation after start-up).  The walk's counters
+ * and tallies live in a static device array of fixed size, one copy per
+ * device, held until the process ends; its address is looked up once per
+ * device when the context is created (tiny_mc_gpu_init or the first call).
+ * Results are bit-identical to tiny_mc_gpu_run_ex with the same arguments.
+ * Returns TMC_E_TOO_LARGE for p->shells > TMC_NOALLOC_MAX_SHELLS.  Every
+ * call on a device uses that device's one array, which keeps the last
+ * call's contents (each call clears what it uses first); the library
+ * serialises its calls, so calls from several threads of a process wait for
+ * each other, and code that bypassed that lock would race on the array. */
+int tiny_mc_gpu_run_noalloc(const tmc_params* p, uint64_t seed, const tmc_options* opt, uint64_t* tally,
+                            tmc_run_stats* stats);
 
 /* Converts raw tallies (as from tiny_mc_gpu_run_ex) into heat / heat2 in
  * the reference's units, in double (host only, no device work). */

@@ -49,6 +49,8 @@ from ._lib import (  # noqa: F401
     philox,
     run,
     run_raw,
+    run_raw_noalloc,
+    NOALLOC_MAX_SHELLS,
     shutdown,
     tally_to_heat,
 )

@@ -34,6 +34,7 @@ _STATUS = {
     -4: "out of memory",
     -5: "photon index range exceeds 2^63",
     -6: "unsupported option",
+    -7: "more shells than the no-allocation scratch holds",
     -8: "request cancelled",
     -9: "request deadline passed",
     -10: "server close timed out",
@@ -41,6 +42,7 @@ _STATUS = {
 E_CANCELLED = -8
 E_DEADLINE = -9
 E_TIMEOUT = -10
+NOALLOC_MAX_SHELLS = 4096
 
 
 class TmcError(RuntimeError):
@@ -137,6 +139,7 @@ def load():
     lib.tmc_options_default.restype = None
     lib.tiny_mc_gpu_run.argtypes = [P(_Params), ctypes.c_uint64, dp, dp]
     lib.tiny_mc_gpu_run_ex.argtypes = [P(_Params), ctypes.c_uint64, P(_Options), u64p, P(_Stats)]
+    lib.tiny_mc_gpu_run_noalloc.argtypes = [P(_Params), ctypes.c_uint64, P(_Options), u64p, P(_Stats)]
     lib.tiny_mc_gpu_tally_to_heat.argtypes = [P(_Params), u64p, dp, dp]
     lib.tiny_mc_gpu_init.argtypes = [ctypes.c_int]
     lib.tiny_mc_gpu_shutdown.argtypes = []
@@ -154,7 +157,7 @@ def load():
     lib.tiny_mc_gpu_check_shell_index.argtypes = [P(_Params), ctypes.c_uint32, ctypes.c_uint64, u64p,
                                                   P(ctypes.c_uint32), u64p]
     lib.tiny_mc_gpu_eval_step_math.argtypes = [ctypes.c_uint32, fp, ctypes.c_uint32, fp, fp, fp, fp]
-    for name in ("tiny_mc_gpu_run", "tiny_mc_gpu_run_ex", "tiny_mc_gpu_tally_to_heat", "tiny_mc_gpu_init",
+    for name in ("tiny_mc_gpu_run", "tiny_mc_gpu_run_ex", "tiny_mc_gpu_run_noalloc", "tiny_mc_gpu_tally_to_heat", "tiny_mc_gpu_init",
                  "tiny_mc_gpu_device_info", "tiny_mc_gpu_philox", "tiny_mc_gpu_eval_math", "tiny_mc_gpu_check_sqrt",
                  "tiny_mc_gpu_eval_step_math", "tiny_mc_gpu_check_short_op", "tiny_mc_gpu_check_shell_index"):
         getattr(lib, name).restype = ctypes.c_int
@@ -200,6 +203,14 @@ def run_raw(params: Params, seed: int, *, sampler=SAMPLER_ANALYTIC, first_photon
     """Integer tallies (uint64 array of 4*shells: 128-bit (lo, hi) sums) and RunStats."""
     return _run_raw("tiny_mc_gpu_run_ex", params, seed, sampler, first_photon, max_interactions, block, grid, chunk,
                     precision, math)
+
+
+def run_raw_noalloc(params: Params, seed: int, *, sampler=SAMPLER_ANALYTIC, first_photon=0, max_interactions=0,
+                    block=0, grid=0, chunk=0, precision=PRECISION_FP32, math=MATH_HARDWARE):
+    """run_raw through tiny_mc_gpu_run_noalloc (static device scratch, no
+    allocation; params.shells <= NOALLOC_MAX_SHELLS)."""
+    return _run_raw("tiny_mc_gpu_run_noalloc", params, seed, sampler, first_photon, max_interactions, block, grid,
+                    chunk, precision, math)
 
 
 def tally_to_heat(params: Params, tally):

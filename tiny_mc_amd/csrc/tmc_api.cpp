@@ -73,12 +73,16 @@ int create_context(int device, Context** out)
     c->compute_units = prop.multiProcessorCount;
     std::snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
     /* default-priority stream (never a prioritised one); no device memory
-     * yet: the single-run words are allocated at the first run */
+     * yet: the single-run words are allocated at the first run, so that the
+     * no-allocation entry point can create the context too */
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess
         || hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
         destroy_context(c);
         return TMC_E_HIP;
     }
+    /* the static scratch's address on this device; a failure here is
+     * reported by tiny_mc_gpu_run_noalloc, the only user */
+    (void)tmc::scratch_setup(device);
     *out = c;
     return TMC_OK;
 }
@@ -482,6 +486,7 @@ TMC_EXPORT const char* tiny_mc_gpu_strerror(int status)
     case TMC_E_NOMEM: return "out of memory";
     case TMC_E_RANGE: return "photon index range exceeds 2^63";
     case TMC_E_UNSUPPORTED: return "unsupported option";
+    case TMC_E_TOO_LARGE: return "more shells than the no-allocation scratch holds";
     case TMC_PENDING: return "pending";
     default: return "unknown status";
     }

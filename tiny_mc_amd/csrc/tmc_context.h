@@ -44,6 +44,10 @@ int acquire_context(Context** out);
 int run_walk(Context* c, const tmc_params* p, uint64_t seed, const tmc_options* o, unsigned long long* work,
              uint64_t* tally, tmc_run_stats* st);
 
+/* The static scratch of tiny_mc_gpu_run_noalloc on `device` (current):
+ * looks its address up once per device.  TMC_OK or a negative status. */
+int scratch_setup(int device);
+
 /* Parameter validation shared by every entry point (TMC_OK or
  * TMC_E_INVALID). */
 int check_params(const tmc_params* p);

@@ -271,6 +271,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--photons", type=int, default=1 << 30)
     ap.add_argument("--sampler", choices=["analytic", "rejection"], default="analytic")
+    ap.add_argument("--weight-table", type=int, default=0,
+                    help="tmc_options.weight_table: 0 automatic, -1 off, N at most N entries")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-port-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -300,12 +302,13 @@ def main():
     name, cus = tiny_mc_amd.device_info()
 
     progress(f"device {name} ({cus} CUs); plain walk: {args.warmup} warmup + {args.steps} timed steps")
+    wt = tiny_mc_amd.WEIGHT_TABLE_OFF if args.weight_table < 0 else args.weight_table
     for i in range(args.warmup):
-        tiny_mc_amd.run_raw(params, 1000 + i, sampler=sampler)
+        tiny_mc_amd.run_raw(params, 1000 + i, sampler=sampler, weight_table=wt)
     kernel_ms, interactions = [], 0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        _, st = tiny_mc_amd.run_raw(params, i + 1, sampler=sampler)
+        _, st = tiny_mc_amd.run_raw(params, i + 1, sampler=sampler, weight_table=wt)
         kernel_ms.append(st.kernel_ms)
         interactions += st.interactions
     elapsed = time.perf_counter() - t0  # every call is synchronous

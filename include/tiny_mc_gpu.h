@@ -92,7 +92,17 @@ typedef struct tmc_options {
     uint32_t precision; /* TMC_PRECISION_* */
     uint32_t math; /* TMC_MATH_* (binary32 walk of TMC_PRECISION_FP32; the other
                       precision modes use the own polynomials or binary64) */
+    uint32_t weight_table; /* binary32 walk: the weight of a surviving photon is a function of
+                              its absorption count, so the walk can read its fixed-point images
+                              from a per-workgroup table instead of computing them (same bits).
+                              0 = automatic (analytic sampler, runs of 2^20 photons or more whose
+                              table spans the first roulette cycle and eight more, up to 1024
+                              entries);
+                              TMC_WEIGHT_TABLE_OFF = never; N = always, at most N entries */
+    uint32_t reserved; /* 0 */
 } tmc_options;
+
+#define TMC_WEIGHT_TABLE_OFF 0xFFFFFFFFu
 
 typedef struct tmc_run_stats {
     uint64_t photons; /* photons walked */
@@ -104,6 +114,7 @@ typedef struct tmc_run_stats {
     uint32_t sampler;
     uint32_t precision;
     uint32_t math;
+    uint32_t weight_table; /* entries of the weight table used (0: none) */
 } tmc_run_stats;
 
 /* Default options (analytic sampler, automatic launch shape, photon 0). */

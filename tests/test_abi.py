@@ -75,11 +75,11 @@ def test_strerror_and_defaults(tmc):
     assert lib.tiny_mc_gpu_strerror(12345) == b"unknown status"
     from tiny_mc_amd._lib import _Options
 
-    o = _Options(7, 7, 7, 7, 7, 7, 7, 7)
+    o = _Options(7, 7, 7, 7, 7, 7, 7, 7, 7, 7)
     lib.tmc_options_default(ctypes.byref(o))
     assert (o.sampler, o.block, o.grid, o.chunk, o.first_photon, o.max_interactions, o.precision,
-            o.math) == (0, 0, 0, 0, 0, 0, 0, 0)
-    assert ctypes.sizeof(o) == 40
+            o.math, o.weight_table, o.reserved) == (0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    assert ctypes.sizeof(o) == 48
 
 
 def test_tally_to_heat_matches_independent_conversion(tmc):

@@ -50,6 +50,39 @@ def test_tallies_bit_exact_vs_mirror(tmc, orc, hw_tables, math, sampler, phys, s
     np.testing.assert_array_equal(tally, want)
 
 
+@pytest.mark.parametrize("math", [0, 1])
+@pytest.mark.parametrize("sampler", [0, 1])
+@pytest.mark.parametrize("phys,photons,table", [
+    (DEFAULT, 1 << 20, 0),          # the automatic table (649 entries at default physics)
+    (DEFAULT, 1 << 16, 80),         # forced, ends 7 absorptions into the second cycle: survivors fall back
+    (LOWALB, 1 << 18, 12),          # albedo 0.5: k1 = 10, the table ends inside the second cycle
+    (HIGHALB, 1 << 14, 1024),       # k1 = 695: survivors of two roulettes run past the table
+])
+def test_weight_table_bit_exact(tmc, orc, hw_tables, math, sampler, phys, photons, table):
+    """The walk that reads its weight images from the LDS table equals the
+    mirror (which multiplies) bit for bit, also for photons that outlive
+    the table and continue with the arithmetic from its last weight."""
+    tally, st = tmc.run_raw(_params(tmc, phys, photons), 606, sampler=sampler, math=math, weight_table=table)
+    assert st.weight_table == (table or (649 if sampler == 0 else 0))
+    tables = hw_tables if math == tmc.MATH_HARDWARE else None
+    want, n = _mirror(orc, phys, 606, photons, sampler=sampler, tables=tables)
+    assert st.interactions == n
+    np.testing.assert_array_equal(tally, want)
+
+
+def test_weight_table_choice(tmc):
+    """Automatic table: on for analytic-sampler runs of 2^20 photons or more
+    whose table spans nine roulette cycles within 1024 entries, off
+    otherwise; never for the other precision modes."""
+    big = 1 << 20
+    assert tmc.run_raw(_params(tmc, DEFAULT, big), 1)[1].weight_table == 73 + 24 * 24
+    assert tmc.run_raw(_params(tmc, DEFAULT, big - 64), 1)[1].weight_table == 0
+    assert tmc.run_raw(_params(tmc, HIGHALB, big), 1, max_interactions=64)[1].weight_table == 0
+    assert tmc.run_raw(_params(tmc, DEFAULT, big), 1, weight_table=tmc.WEIGHT_TABLE_OFF)[1].weight_table == 0
+    assert tmc.run_raw(_params(tmc, DEFAULT, big), 1, sampler=tmc.SAMPLER_REJECTION)[1].weight_table == 0
+    assert tmc.run_raw(_params(tmc, DEFAULT, 4096), 1, precision=tmc.PRECISION_FP64, weight_table=64)[1].weight_table == 0
+
+
 @pytest.mark.parametrize("block,grid,chunk", [(64, 1, 64), (256, 7, 128), (1024, 0, 4096), (512, 3000, 64)])
 def test_launch_shape_independence(tmc, orc, hw_tables, block, grid, chunk):
     p = _params(tmc, DEFAULT, 50000)

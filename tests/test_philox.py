@@ -98,3 +98,34 @@ int main(void) {
     subprocess.run(["gcc", "-std=c11", "-O2", "-I", inc, "-o", str(exe), str(src)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
     assert out.strip() == "0"
+
+
+def test_walk_block_forms_equal_full_generator(tmp_path):
+    """The walk evaluates block (k, b) of a photon from its precomputed part
+    of round 1 (tmc_philox4x32_10_block0, _block0_ks and, with any block
+    number b, _blockb with stored or formed round keys); each must equal the
+    full generator on the same counter and key.  A form that dropped the
+    block number, the photon's counter words or a round key would differ."""
+    src = tmp_path / "forms.c"
+    src.write_text(
+        '#include <stdio.h>\n#include "tmc_philox.h"\n'
+        "static unsigned long long s = 88172645463325252ull;\n"
+        "static unsigned rnd(void){ s ^= s << 13; s ^= s >> 7; s ^= s << 17; return (unsigned)s; }\n"
+        "static int same(tmc_u32x4 a, tmc_u32x4 b){ return a.x0==b.x0 && a.x1==b.x1 && a.x2==b.x2 && a.x3==b.x3; }\n"
+        "int main(void){ long bad = 0;\n"
+        "  for (int i = 0; i < 200000; ++i) {\n"
+        "    unsigned k = i % 3 ? rnd() : (unsigned)(i % 1000), b = i % 5 ? (unsigned)(i % 40) : rnd();\n"
+        "    unsigned c2 = rnd(), c3 = rnd(), k0 = rnd(), k1 = rnd();\n"
+        "    tmc_philox_photon pp = tmc_philox_photon_init(c2, c3, k0, k1);\n"
+        "    tmc_philox_keys ks = tmc_philox_key_schedule(k0, k1);\n"
+        "    tmc_u32x4 f0 = tmc_philox4x32_10(k, 0, c2, c3, k0, k1), fb = tmc_philox4x32_10(k, b, c2, c3, k0, k1);\n"
+        "    bad += !same(f0, tmc_philox4x32_10_block0(k, pp, k0, k1));\n"
+        "    bad += !same(f0, tmc_philox4x32_10_block0_ks(k, pp, ks));\n"
+        "    bad += !same(fb, tmc_philox4x32_10_blockb(k, b, pp, &ks, k0, k1));\n"
+        "    bad += !same(fb, tmc_philox4x32_10_blockb(k, b, pp, 0, k0, k1));\n"
+        "    bad += same(fb, tmc_philox4x32_10_blockb(k, b ^ 1u, pp, &ks, k0, k1));\n"
+        "  }\n  printf(\"%ld\\n\", bad); return 0; }\n")
+    exe = tmp_path / "forms"
+    subprocess.run(["gcc", "-std=c11", "-O2", "-I", os.path.join(ROOT, "tiny_mc_amd", "csrc"), "-o", str(exe),
+                    str(src)], check=True)
+    assert subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.strip() == "0"

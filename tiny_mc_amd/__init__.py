@@ -50,6 +50,7 @@ from ._lib import (  # noqa: F401
     run,
     run_raw,
     run_raw_noalloc,
+    WEIGHT_TABLE_OFF,
     NOALLOC_MAX_SHELLS,
     shutdown,
     tally_to_heat,

@@ -73,7 +73,12 @@ class _Options(ctypes.Structure):
         ("max_interactions", ctypes.c_uint64),
         ("precision", ctypes.c_uint32),
         ("math", ctypes.c_uint32),
+        ("weight_table", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
+
+
+WEIGHT_TABLE_OFF = 0xFFFFFFFF
 
 
 class _Stats(ctypes.Structure):
@@ -89,6 +94,7 @@ class _Stats(ctypes.Structure):
         ("sampler", ctypes.c_uint32),
         ("precision", ctypes.c_uint32),
         ("math", ctypes.c_uint32),
+        ("weight_table", ctypes.c_uint32),
     ]
 
 
@@ -119,6 +125,7 @@ class RunStats:
     sampler: int
     precision: int
     math: int
+    weight_table: int = 0
 
 
 _lib = None
@@ -185,32 +192,33 @@ def run(params: Params, seed: int):
     return heat, heat2
 
 
-def _run_raw(entry, params, seed, sampler, first_photon, max_interactions, block, grid, chunk, precision, math):
+def _run_raw(entry, params, seed, sampler, first_photon, max_interactions, block, grid, chunk, precision, math,
+             weight_table):
     lib = load()
     tally = np.zeros(4 * params.shells, dtype=np.uint64)
-    opt = _Options(sampler, block, grid, chunk, first_photon, max_interactions, precision, math)
+    opt = _Options(sampler, block, grid, chunk, first_photon, max_interactions, precision, math, weight_table, 0)
     st = _Stats()
     c = params._c()
     _check(getattr(lib, entry)(ctypes.byref(c), seed, ctypes.byref(opt), _ptr(tally, ctypes.c_uint64),
                                ctypes.byref(st)), entry)
     stats = RunStats(st.photons, st.interactions, st.truncated, st.kernel_ms, st.total_ms, st.grid, st.block,
-                     st.chunk, st.sampler, st.precision, st.math)
+                     st.chunk, st.sampler, st.precision, st.math, st.weight_table)
     return tally, stats
 
 
 def run_raw(params: Params, seed: int, *, sampler=SAMPLER_ANALYTIC, first_photon=0, max_interactions=0,
-            block=0, grid=0, chunk=0, precision=PRECISION_FP32, math=MATH_HARDWARE):
+            block=0, grid=0, chunk=0, precision=PRECISION_FP32, math=MATH_HARDWARE, weight_table=0):
     """Integer tallies (uint64 array of 4*shells: 128-bit (lo, hi) sums) and RunStats."""
     return _run_raw("tiny_mc_gpu_run_ex", params, seed, sampler, first_photon, max_interactions, block, grid, chunk,
-                    precision, math)
+                    precision, math, weight_table)
 
 
 def run_raw_noalloc(params: Params, seed: int, *, sampler=SAMPLER_ANALYTIC, first_photon=0, max_interactions=0,
-                    block=0, grid=0, chunk=0, precision=PRECISION_FP32, math=MATH_HARDWARE):
+                    block=0, grid=0, chunk=0, precision=PRECISION_FP32, math=MATH_HARDWARE, weight_table=0):
     """run_raw through tiny_mc_gpu_run_noalloc (static device scratch, no
     allocation; params.shells <= NOALLOC_MAX_SHELLS)."""
     return _run_raw("tiny_mc_gpu_run_noalloc", params, seed, sampler, first_photon, max_interactions, block, grid,
-                    chunk, precision, math)
+                    chunk, precision, math, weight_table)
 
 
 def tally_to_heat(params: Params, tally):

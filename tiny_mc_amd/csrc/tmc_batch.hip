@@ -502,7 +502,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
          * around the step) */
         more = live != 0 || (flags & (PENDING | QUEUE_DONE)) != QUEUE_DONE;
         if (live != 0) {
-            absorbed += (uint32_t)__popcll(live);
+            absorbed += (uint32_t)__popcll(SAMPLER == 1u ? __ballot(ph.weight >= 0 && absorbs<SAMPLER>(ph)) : live);
             ++steps;
 
             /* the loop has one back-edge (no continue): with several, the

@@ -71,6 +71,15 @@ float derived_spm(const tmc_params* p);
  * roulette of photon.c:45-49, capped at kcap (scheduling estimate). */
 double expected_interactions(const tmc_params* p, uint64_t kcap);
 
+/* The binary32 weight cadence: absorptions k1 of the first cycle and k2 of
+ * the second (false if the first is longer than `limit`). */
+bool weight_cadence(const tmc_params* p, uint32_t limit, uint32_t* k1, uint32_t* k2);
+
+/* Entries of the single run's weight table for these parameters and the
+ * caller's request (tmc_options.weight_table; 0: none). */
+uint32_t weight_table_entries(const tmc_params* p, uint32_t precision, uint32_t sampler, uint64_t photons,
+                              uint32_t request);
+
 /* Releases of device resources while a serving kernel may be resident.
  * hipFree and hipHostFree synchronise the whole device, so under load they
  * would wait for a resident serving kernel that never becomes idle.  A

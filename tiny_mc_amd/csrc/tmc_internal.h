@@ -21,7 +21,7 @@ struct WalkArgs {
     uint32_t chunk; /* photons claimed per wave at a time (multiple of 64) */
     uint32_t flush_interval; /* wave steps between LDS tally flushes */
     uint32_t kcap; /* stop a photon after kcap absorptions */
-    uint32_t pad;
+    uint32_t table_k; /* entries of the weight table built in LDS (binary32 walk); 0: none */
     unsigned long long* counters; /* [0] claim offset, [1] interactions, [2] truncated */
     unsigned long long* tally; /* 4*shells words of 128-bit sums (lo, hi) */
     /* precision variants */
@@ -33,14 +33,18 @@ struct WalkArgs {
     uint32_t pad2;
 };
 
+/* Largest weight table of the binary32 walk (entries of 16 bytes in LDS). */
+#define TMC_WALK_TABLE_MAX 1024u
+
 /* Launches the persistent walk kernel of `precision` (TMC_PRECISION_*) on
- * `stream`.  LDS: 16 bytes per shell plus the overflow shell's lane copies. */
+ * `stream`.  LDS: 16 bytes per shell plus the overflow shell's lane copies,
+ * and a.table_k entries of the weight table (TMC_PRECISION_FP32 only). */
 hipError_t launch_walk(uint32_t sampler, uint32_t precision, uint32_t math, const WalkArgs& a, uint32_t grid,
                        uint32_t block, hipStream_t stream);
 
 /* Resident workgroups per CU for the walk kernel at this block size. */
 hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t math, uint32_t block, uint32_t shells,
-                          int* wg_per_cu);
+                          uint32_t table_k, int* wg_per_cu);
 
 /* Zeroes n 64-bit words at p on `stream` (a kernel, so the sequence holds no
  * memset node). */

@@ -149,6 +149,34 @@ TMC_HD tmc_u32x4 tmc_philox4x32_10_block0_ks(uint32_t k, tmc_philox_photon pp, t
     return r;
 }
 
+/* Block (k, b) from the photon's part of round 1: counter word 1 enters
+ * round 1 only through c0' = hi(c2 M1) ^ b ^ key0, so block b costs what
+ * block 0 does plus one exclusive or.  Round keys from `ks` (STORED) or
+ * formed from key0/key1. */
+TMC_HD tmc_u32x4 tmc_philox4x32_10_blockb(uint32_t k, uint32_t b, tmc_philox_photon pp, const tmc_philox_keys* ks,
+                                          uint32_t k0, uint32_t k1)
+{
+    const uint64_t q0 = (uint64_t)k * TMC_PHILOX_M0;
+    uint32_t c0 = pp.c0r1 ^ b, c1 = pp.c1r1, c2 = (uint32_t)(q0 >> 32) ^ pp.c3k1, c3 = (uint32_t)q0;
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int round = 1; round < 10; ++round) {
+        const uint32_t rk0 = ks ? ks->k0[round] : k0 + (uint32_t)round * TMC_PHILOX_W0;
+        const uint32_t rk1 = ks ? ks->k1[round] : k1 + (uint32_t)round * TMC_PHILOX_W1;
+        const uint64_t p0 = (uint64_t)c0 * TMC_PHILOX_M0;
+        const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
+        const uint32_t n0 = tmc_xor3((uint32_t)(p1 >> 32), c1, rk0);
+        const uint32_t n2 = tmc_xor3((uint32_t)(p0 >> 32), c3, rk1);
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    tmc_u32x4 r = { c0, c1, c2, c3 };
+    return r;
+}
+
 /* 32 random bits -> uniform in (0, 1]: ((r >> 8) + 1) * 2^-24, exact in
  * binary32.  The reference maps rand() to [0, 1] (photon.c:21); excluding 0
  * keeps -log(xi) finite, which the reference does not guarantee. */

@@ -18,6 +18,10 @@ struct Photon {
     uint32_t k; /* absorptions so far = Philox counter word 0 */
     uint32_t plo, phi; /* photon index = Philox counter words 2-3 */
     tmc_philox_photon pp; /* the photon's part of Philox round 1 */
+    /* rejection sampler: direction attempts made for absorption k so far
+     * (0: none; the absorption itself is done once this is nonzero), with
+     * bit 31 the absorption's cutoff flag, and its roulette word */
+    uint32_t att, x3;
 };
 
 /* Physics of the photon's parameter set. */
@@ -96,6 +100,8 @@ __device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon, uint32
     p.w = 1.0f;
     p.weight = 1.0f;
     p.k = 0;
+    p.att = 0;
+    p.x3 = 0;
     p.plo = (uint32_t)photon;
     p.phi = (uint32_t)(photon >> 32);
     p.pp = tmc_philox_photon_init(p.plo, p.phi, key0, key1);
@@ -122,6 +128,22 @@ __device__ __forceinline__ void tally_add(const LaneTally& lt, bool inner, uint3
     atomicAdd(p + 1, b);
 }
 
+/* The binary32 weight sequence of one parameter set, as a table in LDS.
+ * A photon's weight is a function of its absorption count k alone: every
+ * photon starts at 1, multiplies by the albedo at each absorption
+ * (photon.c:32) and, once below the cutoff, either dies or continues at the
+ * weight divided by 0.1 (photon.c:45-49), the same for every survivor.
+ * Entry k holds the fixed-point images of the weight W[k] deposited at
+ * absorption k (fx_heat, fx_heat2) and, in bit 31 of word 1, whether
+ * RN(W[k] a) falls below the cutoff.  Photons at k >= k_end (rare: the table
+ * spans many roulette cycles) fall back to the arithmetic, starting from
+ * w_end = W[k_end]. */
+struct WeightTable {
+    const uint4* entry;
+    uint32_t k_end;
+    float w_end;
+};
+
 /* Whether a live photon of the wave can reach the absorption cap kcap
  * within the next `steps` steps: its k grows by one per step, and photons
  * launched meanwhile start at k = 0.  Evaluated at each periodic flush, it
@@ -147,47 +169,117 @@ __device__ __forceinline__ bool cap_within(const PhotonT& ph, uint32_t kcap, uin
  * arithmetic in both modes.  STORED_KEYS takes the Philox round keys from
  * ph.keys; otherwise they are formed from the seed inside the step (fewer
  * live registers, for kernels short of scalar registers). */
-template <uint32_t SAMPLER, uint32_t MATH, bool STORED_KEYS = true>
-__device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt)
+/* Whether the walk's next iteration of p is an absorption (the rejection
+ * sampler's retries of a direction are iterations without one). */
+template <uint32_t SAMPLER>
+__device__ __forceinline__ bool absorbs(const Photon& p)
 {
-    const tmc_u32x4 r = STORED_KEYS ? tmc_philox4x32_10_block0_ks(p.k, p.pp, ph.keys) /* block (k, 0) */
-                                    : tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1);
+    return SAMPLER == 0u || p.att == 0u;
+}
 
-    /* move (photon.c:21-24) */
-    const float t = step_length<MATH>(r.x0);
-    p.x = __builtin_fmaf(t, p.u, p.x);
-    p.y = __builtin_fmaf(t, p.v, p.y);
-    p.z = __builtin_fmaf(t, p.w, p.z);
+/* Walks one iteration of p: with the analytic sampler, and with the
+ * rejection sampler when the first attempt is accepted, one absorption
+ * event, its deposit added to the fixed-point shell sums in LDS.  Returns
+ * false when the roulette kills the photon.
+ * MATH (TMC_MATH_*) selects how -ln u, sin/cos(2 pi u') and the analytic
+ * sampler's sin(theta) are evaluated; everything else is the same
+ * arithmetic in both modes.  STORED_KEYS takes the Philox round keys from
+ * ph.keys; otherwise they are formed from the seed inside the step (fewer
+ * live registers, for kernels short of scalar registers).  TABLE reads the
+ * weight's images from wt (WeightTable).
+ * Rejection sampler (photon.c:34-43): attempt 1 uses words 1-2 of block
+ * (k, 0), attempts 2 and 3 words 0-1 and 2-3 of block (k, 1), and so on.
+ * An iteration makes at most the attempts of one block: a photon whose
+ * attempt is rejected keeps its absorption pending (position, deposit and
+ * weight done, direction, k and roulette not) and makes the next attempts
+ * in the next iteration, from the next block.  Its lanes thus compute one
+ * block per iteration like every other lane, instead of the whole wave
+ * computing a second block whenever one of its 64 lanes rejects (almost
+ * every time: 1 - (pi/4)^64).  The arithmetic and its order per absorption
+ * are unchanged. */
+template <uint32_t SAMPLER, uint32_t MATH, bool STORED_KEYS = true, bool TABLE = false>
+__device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt,
+                                     const WeightTable& wt = WeightTable{ nullptr, 0u, 0.0f })
+{
+    const bool fresh = absorbs<SAMPLER>(p);
+    /* the table entry first: its load overlaps the generator */
+    uint4 e = { 0u, 0u, 0u, 0u };
+    if (TABLE) {
+        e = wt.entry[p.k < wt.k_end ? p.k : wt.k_end - 1u];
+        __builtin_amdgcn_sched_barrier(0); /* keep the load ahead of the generator */
+    }
+    tmc_u32x4 r;
+    if (SAMPLER == 1u) /* block (k, 0) for an absorption, (k, b) for attempts 2b and 2b + 1 */
+        r = tmc_philox4x32_10_blockb(p.k, (p.att & 0xFFu) ? ((p.att & 0xFFu) + 1u) >> 1 : 0u, p.pp,
+                                     STORED_KEYS ? &ph.keys : nullptr, ph.key0, ph.key1);
+    else
+        r = STORED_KEYS ? tmc_philox4x32_10_block0_ks(p.k, p.pp, ph.keys) /* block (k, 0) */
+                        : tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1);
 
-    /* absorb (photon.c:26-32) */
-    const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
-    bool inner; /* else the overflow shell */
-    const uint32_t shell = shell_index(r2, ph, &inner);
-    tally_add(lt, inner, shell, fx_heat(p.weight), fx_heat2(p.weight));
-    p.weight = p.weight * ph.albedo;
+    bool cut = false; /* the weight after this absorption is below the cutoff */
+    if (fresh) {
+        /* move (photon.c:21-24) */
+        const float t = step_length<MATH>(r.x0);
+        p.x = __builtin_fmaf(t, p.u, p.x);
+        p.y = __builtin_fmaf(t, p.v, p.y);
+        p.z = __builtin_fmaf(t, p.w, p.z);
+
+        /* absorb (photon.c:26-32) */
+        const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
+        bool inner; /* else the overflow shell */
+        const uint32_t shell = shell_index(r2, ph, &inner);
+        if (TABLE) {
+            uint64_t h = ((uint64_t)(e.y & 0x7FFFFFFFu) << 32) | e.x, h2 = ((uint64_t)e.w << 32) | e.z;
+            cut = (int32_t)e.y < 0;
+            if (__builtin_expect(__ballot(p.k >= wt.k_end) != 0, 0)) {
+                /* photons past the table continue with the arithmetic */
+                if (p.k >= wt.k_end) {
+                    if (p.k == wt.k_end)
+                        p.weight = wt.w_end;
+                    h = fx_heat(p.weight);
+                    h2 = fx_heat2(p.weight);
+                    p.weight = p.weight * ph.albedo;
+                    cut = p.weight < 0.001f;
+                }
+            }
+            tally_add(lt, inner, shell, h, h2);
+        } else {
+            tally_add(lt, inner, shell, fx_heat(p.weight), fx_heat2(p.weight));
+            p.weight = p.weight * ph.albedo;
+            cut = p.weight < 0.001f;
+        }
+    }
 
     /* new direction (photon.c:34-43) */
+    uint32_t x3 = r.x3; /* the roulette word of block (k, 0) */
     if (SAMPLER == 1u) {
-        uint32_t w0 = r.x1, w1 = r.x2;
-        tmc_u32x4 blk = { 0u, 0u, 0u, 0u };
-        float xi1, xi2, tt;
-        uint32_t att = 0;
-        for (;;) {
-            xi1 = tmc_u11(w0);
-            xi2 = tmc_u11(w1);
+        uint32_t att = p.att & 0xFFu;
+        float xi1 = tmc_u11(fresh ? r.x1 : r.x0), xi2 = tmc_u11(fresh ? r.x2 : r.x1);
+        float tt = __builtin_fmaf(xi2, xi2, xi1 * xi1);
+        ++att;
+        bool accept = (tt <= 1.0f && tt > 0.0f) || att >= 64u;
+        if (!accept && !fresh) {
+            xi1 = tmc_u11(r.x2);
+            xi2 = tmc_u11(r.x3);
             tt = __builtin_fmaf(xi2, xi2, xi1 * xi1);
             ++att;
-            if ((tt <= 1.0f && tt > 0.0f) || att >= 64u)
-                break;
-            if (att & 1u) {
-                blk = tmc_philox4x32_10(p.k, (att + 1u) >> 1, p.plo, p.phi, ph.key0, ph.key1);
-                w0 = blk.x0;
-                w1 = blk.x1;
-            } else {
-                w0 = blk.x2;
-                w1 = blk.x3;
-            }
+            accept = (tt <= 1.0f && tt > 0.0f) || att >= 64u;
         }
+        if (!accept) {
+            /* pending: the next iteration makes the next attempts */
+            if (fresh) {
+                p.x3 = r.x3;
+                p.att = att | (cut ? 0x80000000u : 0u);
+            } else {
+                p.att = (p.att & 0x80000000u) | att;
+            }
+            return true;
+        }
+        if (!fresh) {
+            cut = (p.att & 0x80000000u) != 0u;
+            x3 = p.x3;
+        }
+        p.att = 0;
         p.u = __builtin_fmaf(2.0f, tt, -1.0f);
         const float s = 2.0f * sqrt_cr_walk(1.0f - tt); /* 1 - tt: 0 or >= 2^-24 */
         p.v = xi1 * s;
@@ -204,10 +296,11 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
     ++p.k;
 
     /* roulette (photon.c:45-49) */
-    if (p.weight < 0.001f) {
-        if (r.x3 >= TMC_ROULETTE_KILL_R) /* u > 0.1f */
+    if (cut) {
+        if (x3 >= TMC_ROULETTE_KILL_R) /* u > 0.1f */
             return false;
-        p.weight = div_tenth(p.weight); /* = p.weight / 0.1f */
+        if (!TABLE || p.k > wt.k_end) /* the weight is the arithmetic's (p.k counts this absorption) */
+            p.weight = div_tenth(p.weight); /* = p.weight / 0.1f */
     }
     return true;
 }

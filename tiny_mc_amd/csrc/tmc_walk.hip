@@ -38,8 +38,47 @@ namespace tmc {
 #define TMC_WALK_HOT 32u
 
 /* LDS words of one workgroup: w and w*w sums per shell, then the
- * overflow shell's copies of both. */
+ * overflow shell's copies of both (a multiple of 2 words: the table that
+ * follows is 16-byte aligned). */
 __host__ __device__ __forceinline__ size_t walk_lds_words(uint32_t shells) { return 2u * shells + 2u * TMC_WALK_HOT; }
+
+/* LDS bytes of a workgroup: the tallies, then table_k table entries (16
+ * bytes) and the weight sequence they are built from (table_k + 1 floats). */
+__host__ __device__ __forceinline__ size_t walk_lds_bytes(uint32_t shells, uint32_t table_k)
+{
+    return walk_lds_words(shells) * sizeof(unsigned long long)
+        + (table_k ? 16u * (size_t)table_k + 4u * ((size_t)table_k + 1u) : 0u);
+}
+
+/* Builds the weight table of a binary32 walk in LDS (WeightTable): one
+ * thread runs the recurrence W[0] = 1, W[k+1] = RN(W[k] a), divided by 0.1
+ * when it falls below the cutoff (the arithmetic of step()), then every
+ * thread converts its entries.  Ends with a barrier. */
+__device__ __forceinline__ WeightTable build_weight_table(unsigned long long* lds, uint32_t shells, uint32_t K,
+                                                          float albedo)
+{
+    uint4* tab = (uint4*)(lds + walk_lds_words(shells));
+    float* wseq = (float*)(tab + K);
+    if (threadIdx.x == 0) {
+        float w = 1.0f;
+        for (uint32_t k = 0; k < K; ++k) {
+            wseq[k] = w;
+            const float w2 = w * albedo;
+            w = w2 < 0.001f ? div_tenth(w2) : w2;
+        }
+        wseq[K] = w;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
+        const float w = wseq[k];
+        const uint64_t h = fx_heat(w), h2 = fx_heat2(w);
+        const bool cut = w * albedo < 0.001f;
+        tab[k] = make_uint4((uint32_t)h, (uint32_t)(h >> 32) | (cut ? 0x80000000u : 0u), (uint32_t)h2,
+                            (uint32_t)(h2 >> 32));
+    }
+    __syncthreads();
+    return WeightTable{ tab, K, wseq[K] };
+}
 
 /* Moves this workgroup's LDS tallies (shell pairs, then the overflow
  * shell's copies, each a (w image, w*w image) pair of words) into the global
@@ -72,33 +111,55 @@ __device__ __forceinline__ void flush_tallies(unsigned long long* lds, unsigned 
  * one of the precision variants (tmc_step_prec.h). */
 /* VARIANT: the math mode for the binary32 walk, compensation for the
  * weight-from-counts walk, unused for binary64. */
-template <uint32_t SAMPLER, uint32_t VARIANT>
-__device__ __forceinline__ bool step_of(Photon& p, const Physics& ph, const LaneTally& lt)
+template <uint32_t SAMPLER, uint32_t VARIANT, bool TABLE>
+__device__ __forceinline__ bool step_of(Photon& p, const Physics& ph, const LaneTally& lt, const WeightTable& wt)
 {
-    return step<SAMPLER, VARIANT>(p, ph, lt);
+    return step<SAMPLER, VARIANT, true, TABLE>(p, ph, lt, wt);
 }
 
-template <uint32_t SAMPLER, uint32_t VARIANT>
-__device__ __forceinline__ bool step_of(PhotonW& p, const PhysicsW& ph, const LaneTally& lt)
+template <uint32_t SAMPLER, uint32_t VARIANT, bool TABLE>
+__device__ __forceinline__ bool step_of(PhotonW& p, const PhysicsW& ph, const LaneTally& lt, const WeightTable&)
 {
     return step<SAMPLER, VARIANT != 0u>(p, ph, lt);
 }
 
-template <uint32_t SAMPLER, uint32_t VARIANT>
-__device__ __forceinline__ bool step_of(Photon64& p, const Physics64& ph, const LaneTally& lt)
+template <uint32_t SAMPLER, uint32_t VARIANT, bool TABLE>
+__device__ __forceinline__ bool step_of(Photon64& p, const Physics64& ph, const LaneTally& lt, const WeightTable&)
 {
     return step<SAMPLER>(p, ph, lt);
 }
 
+/* Whether the next iteration of a live photon is an absorption (only the
+ * binary32 walk's rejection sampler has iterations that are not). */
+template <uint32_t SAMPLER>
+__device__ __forceinline__ bool absorbs_of(const Photon& p)
+{
+    return absorbs<SAMPLER>(p);
+}
+template <uint32_t SAMPLER>
+__device__ __forceinline__ bool absorbs_of(const PhotonW&)
+{
+    return true;
+}
+template <uint32_t SAMPLER>
+__device__ __forceinline__ bool absorbs_of(const Photon64&)
+{
+    return true;
+}
+
 /* The persistent walk over photons [first_photon, first_photon + photons). */
-template <class PhotonT, class PhysicsT, uint32_t SAMPLER, uint32_t VARIANT>
+template <class PhotonT, class PhysicsT, uint32_t SAMPLER, uint32_t VARIANT, bool TABLE = false>
 __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phys)
 {
     extern __shared__ unsigned long long lds_tally[];
     unsigned long long* hot = lds_tally + 2 * a.shells;
     for (uint32_t i = threadIdx.x; i < walk_lds_words(a.shells); i += blockDim.x)
         lds_tally[i] = 0;
-    __syncthreads();
+    WeightTable wt = { nullptr, 0u, 0.0f };
+    if (TABLE)
+        wt = build_weight_table(lds_tally, a.shells, a.table_k, a.albedo); /* ends with a barrier */
+    else
+        __syncthreads();
 
     const uint32_t lane = __lane_id();
     const LaneTally lt = { lds_tally, hot + 2 * (lane & (TMC_WALK_HOT - 1)) };
@@ -148,9 +209,10 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
         const uint64_t live = __ballot(ph.weight >= 0);
         if (live == 0)
             break;
-        wave_interactions += (uint64_t)__popcll(live);
+        wave_interactions += (uint64_t)__popcll(SAMPLER == 1u ? __ballot(ph.weight >= 0 && absorbs_of<SAMPLER>(ph))
+                                                              : live);
 
-        if (ph.weight >= 0 && !step_of<SAMPLER, VARIANT>(ph, phys, lt))
+        if (ph.weight >= 0 && !step_of<SAMPLER, VARIANT, TABLE>(ph, phys, lt, wt))
             ph.weight = -1;
         /* photons that reached the absorption cap stop; checked only while
          * some photon of the wave can reach it before the next flush */
@@ -179,8 +241,9 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
     }
 }
 
-/* The reference's FP32 arithmetic (the production walk). */
-template <uint32_t SAMPLER, uint32_t MATH>
+/* The reference's FP32 arithmetic (the production walk); TABLE: the
+ * weight sequence from the LDS table (WeightTable). */
+template <uint32_t SAMPLER, uint32_t MATH, bool TABLE>
 __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
 {
     Physics phys = {};
@@ -190,7 +253,7 @@ __global__ void __launch_bounds__(1024) walk_f32(WalkArgs a)
     phys.key0 = (uint32_t)a.seed;
     phys.key1 = (uint32_t)(a.seed >> 32);
     set_derived(phys);
-    walk_body<Photon, Physics, SAMPLER, MATH>(a, phys);
+    walk_body<Photon, Physics, SAMPLER, MATH, TABLE>(a, phys);
 }
 
 /* FP32 transport, weight from exact counts (COMP: compensated positions). */
@@ -215,7 +278,7 @@ namespace {
 
 using WalkKernel = void (*)(WalkArgs);
 
-WalkKernel walk_kernel(uint32_t sampler, uint32_t precision, uint32_t math)
+WalkKernel walk_kernel(uint32_t sampler, uint32_t precision, uint32_t math, bool table)
 {
     const bool rej = sampler == 1u;
     switch (precision) {
@@ -223,9 +286,14 @@ WalkKernel walk_kernel(uint32_t sampler, uint32_t precision, uint32_t math)
     case 2u: return rej ? walk_f32w<1u, true> : walk_f32w<0u, true>;
     case 3u: return rej ? walk_f64<1u> : walk_f64<0u>;
     default:
-        if (math == TMC_MATH_OWN)
-            return rej ? walk_f32<1u, TMC_MATH_OWN> : walk_f32<0u, TMC_MATH_OWN>;
-        return rej ? walk_f32<1u, TMC_MATH_HARDWARE> : walk_f32<0u, TMC_MATH_HARDWARE>;
+        if (math == TMC_MATH_OWN) {
+            if (table)
+                return rej ? walk_f32<1u, TMC_MATH_OWN, true> : walk_f32<0u, TMC_MATH_OWN, true>;
+            return rej ? walk_f32<1u, TMC_MATH_OWN, false> : walk_f32<0u, TMC_MATH_OWN, false>;
+        }
+        if (table)
+            return rej ? walk_f32<1u, TMC_MATH_HARDWARE, true> : walk_f32<0u, TMC_MATH_HARDWARE, true>;
+        return rej ? walk_f32<1u, TMC_MATH_HARDWARE, false> : walk_f32<0u, TMC_MATH_HARDWARE, false>;
     }
 }
 
@@ -234,17 +302,18 @@ WalkKernel walk_kernel(uint32_t sampler, uint32_t precision, uint32_t math)
 hipError_t launch_walk(uint32_t sampler, uint32_t precision, uint32_t math, const WalkArgs& a, uint32_t grid,
                        uint32_t block, hipStream_t stream)
 {
-    const size_t lds = walk_lds_words(a.shells) * sizeof(unsigned long long);
-    hipLaunchKernelGGL(walk_kernel(sampler, precision, math), dim3(grid), dim3(block), lds, stream, a);
+    const uint32_t K = precision == 0u ? a.table_k : 0u;
+    hipLaunchKernelGGL(walk_kernel(sampler, precision, math, K != 0u), dim3(grid), dim3(block),
+                       walk_lds_bytes(a.shells, K), stream, a);
     return hipGetLastError();
 }
 
 hipError_t walk_occupancy(uint32_t sampler, uint32_t precision, uint32_t math, uint32_t block, uint32_t shells,
-                          int* wg_per_cu)
+                          uint32_t table_k, int* wg_per_cu)
 {
-    const size_t lds = walk_lds_words(shells) * sizeof(unsigned long long);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_kernel(sampler, precision, math), (int)block,
-                                                        lds);
+    const uint32_t K = precision == 0u ? table_k : 0u;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, walk_kernel(sampler, precision, math, K != 0u),
+                                                        (int)block, walk_lds_bytes(shells, K));
 }
 
 __global__ void clear_kernel(unsigned long long* p, size_t n)

@@ -9,18 +9,24 @@ data: photons are generated from (seed, photon index) on the device.
 Prints one JSON line: whole-job photons/s, the roofline of the walk kernel
 (algorithmic FP32 FLOPs of the reference algorithm, 40.5 per absorption
 event counted from photon.c:21-48, over the kernel's average duration
-measured with HIP events on the library's stream), and the reference CPU
+measured with HIP events on the library's stream), the same walk with the
+reference's own rejection sampler (photon.c:34-43), and the reference CPU
 loop timed on this machine's host cores.
 
-Two more legs of the metric follow, each once:
+More legs of the metric follow, each once:
   sweep   4096 parameter sets (mu_a log-uniform [0.05,10]/cm, mu_s
           log-uniform [10,200]/cm, shells {51,101,201,401}, width U[10,100] um,
-          photons log-uniform [2^16,2^22], seed = set index) in one persistent
-          launch, beside the same sets run one launch per set;
+          photons log-uniform [2^16,2^22], seed = set index) in persistent
+          launches with autotuned options (tuned outside the timed runs; three
+          timed runs, and two with the default options), beside the same sets
+          run one launch per set;
   serve   Poisson arrivals of requests (sweep physics, Pareto(1.5) photons in
-          [2^14,2^24]) at a fixed fraction of the sweep's measured capacity;
-          p50/p99 latency, aggregate photons/s, and a sample of results
-          checked bit for bit against the same request run alone.
+          [2^14,2^24]) at half the sweep's measured capacity (latency) and at
+          its full capacity (throughput at saturation), with p50/p99 latency,
+          aggregate photons/s and a sample of results checked bit for bit
+          against the same request run alone; and requests of the saturation
+          stream run one launch per request (the plain path, as the
+          reference's loop of tiny_mc.c:47-49 per configuration).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons N]
                     [--sampler analytic|rejection] [--no-cpu-baseline]
@@ -38,7 +44,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "photons/s at default physics; sweep photons/s + p50/p99 latency at fixed load"
-FLOP_PER_INTERACTION = 40.5  # FP32 FLOPs of photon.c:21-48 per absorption event
+# FP32 FLOPs per absorption event of photon.c:21-48 (add, mul, div, sqrt and
+# log 1 each): 2 step, 6 move, 7 radius and shell, 6 tally and weight,
+# 19.46 direction by rejection (9 per attempt x 4/pi attempts, 2 for u,
+# 6 for v and w), 0.03 roulette.  The algorithmic work the roofline counts.
+FLOP_PER_INTERACTION = 40.5
+# The analytic sampler does the same work but the direction: 2u - 1 (2),
+# 1 - mu^2 (2), sqrt (1), the angle (1), sin and cos (2), v and w (2).
+FLOP_PER_INTERACTION_ANALYTIC = 40.5 - 19.46 + 10.0
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "walk_f32_default_2e30_traffic.txt")
 REF_BENCH = os.path.join(ROOT, "oracle", "_ref", "tiny_mc_bench")
@@ -69,11 +82,14 @@ def progress(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(max_procs=16, photons_per_proc=1 << 19):
+def cpu_baseline(photons_per_proc=1 << 22, max_procs=16):
     """The reference loop (photon() + glibc rand(), unmodified sources built
-    by oracle/Makefile) on the host cores: one process per core, distinct
-    seeds (threads would serialise on rand()'s lock).  Σ photons / wall."""
-    procs = max(1, min(max_procs, len(os.sched_getaffinity(0))))
+    by oracle/Makefile) on the host cores: one process per usable core, at
+    most max_procs (the host cores this machine allots to one GPU), distinct
+    seeds (threads would serialise on rand()'s lock), each about 13 s of
+    work.  Sum of photons / wall time."""
+    usable = len(os.sched_getaffinity(0))
+    procs = max(1, min(max_procs, usable))
     if os.path.exists(REF_BENCH):
         cmd, kind = [REF_BENCH], "reference"
     else:
@@ -85,7 +101,7 @@ def cpu_baseline(max_procs=16, photons_per_proc=1 << 19):
     for i in range(procs):
         env = dict(os.environ, TMC_SEED=str(i + 1), **{env_key: str(photons_per_proc)})
         running.append(subprocess.Popen(cmd, env=env, stdout=subprocess.PIPE, text=True))
-    rates = []
+    rates, secs = [], []
     for p in running:
         out, _ = p.communicate()
         if p.returncode != 0:
@@ -93,14 +109,22 @@ def cpu_baseline(max_procs=16, photons_per_proc=1 << 19):
         m = re.search(r"^# ([0-9.]+) K photons per second", out, re.M)
         if m:
             rates.append(float(m.group(1)) * 1e3)
+        m = re.search(r"^# ([0-9.]+) seconds", out, re.M)
+        if m:
+            secs.append(float(m.group(1)))
     wall = time.perf_counter() - t0
-    shipped = ref_rate(REF_SHIPPED, 1, photons_per_proc // 2) if os.path.exists(REF_SHIPPED) else None
+    shipped = ref_rate(REF_SHIPPED, 1, 1 << 19) if os.path.exists(REF_SHIPPED) else None
     return {
         "value": procs * photons_per_proc / wall,
         "unit": "photons/s",
         "cores": procs,
         "kind": kind,
         "cpu_model": cpu_model(),
+        "cpu_count": os.cpu_count(),
+        "usable_cores": usable,
+        "cores_cap": max_procs,
+        "cores_cap_reason": "host cores allotted to one GPU on this machine",
+        "seconds_per_process": {"min": min(secs), "max": max(secs)} if secs else None,
         "single_core_value": sum(rates) / len(rates) if rates else None,
         "single_core_as_shipped_value": shipped,
         "sample": f"{procs} processes x {photons_per_proc} photons, default physics, seeds 1..{procs}; "
@@ -162,41 +186,67 @@ def rocpd_summary(db_path):
     return "\n".join(lines) + "\n"
 
 
-def sweep_leg():
-    """The batched sweep in one launch vs one launch per set (same sets)."""
+def sweep_leg(tuned_runs=3, default_runs=2):
+    """The batched sweep in persistent launches vs one launch per set (same
+    sets).  Options autotuned on a 1/16 sample before the timed runs."""
+    import statistics
+
     import tiny_mc_amd
 
     sets = tiny_mc_amd.sweep_sets(n=4096, seed=0)
     photons = sum(p.photons for p, _ in sets)
     # warm the multi-set kernel's code object on a small batch
     tiny_mc_amd.batch_run(tiny_mc_amd.sweep_sets(n=8, seed=1, photon_scale=1e-3))
-    with tiny_mc_amd.BatchPlan(sets, tiny_mc_amd.BatchOptions(use_graph=True)) as plan:
-        t0 = time.perf_counter()
-        _, st = plan.run()
-        batched_s = time.perf_counter() - t0
-    progress(f"sweep batched: {batched_s:.2f} s; now one launch per set")
+    t0 = time.perf_counter()
+    tuned = tiny_mc_amd.batch_autotune(sets, tiny_mc_amd.BatchOptions(), sample_divisor=16)
+    tune_s = time.perf_counter() - t0
+    tuned.use_graph = True
+    progress(f"sweep: autotuned in {tune_s:.1f} s: {tuned}")
+
+    def timed(options, runs):
+        out, st = [], None
+        with tiny_mc_amd.BatchPlan(sets, options) as plan:
+            for _ in range(runs):
+                t = time.perf_counter()
+                _, st = plan.run()
+                out.append(time.perf_counter() - t)
+        return out, st
+
+    tuned_s, st = timed(tuned, tuned_runs)
+    progress(f"sweep batched (tuned): {', '.join(f'{x:.2f}' for x in tuned_s)} s")
+    default_s, _ = timed(tiny_mc_amd.BatchOptions(use_graph=True), default_runs)
+    progress(f"sweep batched (defaults): {', '.join(f'{x:.2f}' for x in default_s)} s; now one launch per set")
     t0 = time.perf_counter()
     for p, seed in sets:
         tiny_mc_amd.run_raw(p, seed)
     per_set_s = time.perf_counter() - t0
+    med = statistics.median(tuned_s)
     return {
         "sets": len(sets),
         "photons": photons,
         "interactions": st.interactions,
-        "seconds": batched_s,
-        "photons_per_s": photons / batched_s,
-        "interactions_per_s": st.interactions / batched_s,
+        "seconds": med,
+        "seconds_runs": tuned_s,
+        "seconds_min": min(tuned_s),
+        "photons_per_s": photons / med,
+        "interactions_per_s": st.interactions / med,
+        "default_options_seconds_runs": default_s,
+        "tuned_vs_default": statistics.median(default_s) / med,
         "one_launch_per_set_seconds": per_set_s,
         "one_launch_per_set_photons_per_s": photons / per_set_s,
-        "speedup_vs_one_launch_per_set": per_set_s / batched_s,
+        "speedup_vs_one_launch_per_set": per_set_s / med,
+        "speedup_vs_one_launch_per_set_best": per_set_s / min(tuned_s),
         "chunks": st.chunks,
         "lane_utilisation": st.lane_utilisation,
+        "autotune_seconds": tune_s,
         "launch": {"passes": st.passes, "grid": st.grid, "block": st.block, "waves_per_simd": st.waves_per_simd,
-                   "copies": st.copies, "chunk_work": st.chunk_work, "hip_graph": True},
+                   "copies": st.copies, "chunk_work": st.chunk_work, "hip_graph": True,
+                   "options": {"block": tuned.block, "waves_per_simd": tuned.waves_per_simd, "copies": tuned.copies,
+                               "chunk_work": tuned.chunk_work}},
     }
 
 
-def serve_leg(capacity_interactions_per_s, load=0.5, seconds=15.0, check=16):
+def serve_leg(capacity_interactions_per_s, load=0.5, seconds=15.0, check=16, seed=12, max_chunk_scale=0):
     """Open-loop Poisson load at `load` x capacity for about `seconds`."""
     import numpy as np
 
@@ -206,8 +256,8 @@ def serve_leg(capacity_interactions_per_s, load=0.5, seconds=15.0, check=16):
     mean_work = tiny_mc_amd.serve_load.mean_request_work(probe)
     rate = load * capacity_interactions_per_s / mean_work
     n = max(50, int(rate * seconds))
-    reqs = tiny_mc_amd.serve_load.draw_requests(n, seed=12)
-    with tiny_mc_amd.Server() as srv:
+    reqs = tiny_mc_amd.serve_load.draw_requests(n, seed=seed)
+    with tiny_mc_amd.Server(max_chunk_scale=max_chunk_scale) as srv:
         progress(f"serve: {n} requests at {rate:.4g}/s ({load:.2f} of capacity)")
         recs, wall = tiny_mc_amd.serve_load.run_open_loop(srv, reqs, rate, progress=progress)
         stats = srv.stats()
@@ -229,7 +279,28 @@ def serve_leg(capacity_interactions_per_s, load=0.5, seconds=15.0, check=16):
         "seconds": wall,
         "kernel_launches": stats["kernel_launches"],
         "exact_vs_alone": f"{exact}/{len(sample)}",
-    }
+    }, reqs, recs
+
+
+def one_launch_per_request_leg(reqs, count=1500):
+    """The plain path on the served requests: each request (a random subset
+    of the stream) as one synchronous launch, back to back."""
+    import numpy as np
+
+    import tiny_mc_amd
+
+    rng = np.random.default_rng(7)
+    idx = rng.choice(len(reqs), size=min(count, len(reqs)), replace=False)
+    inter, photons = 0, 0
+    t0 = time.perf_counter()
+    for i in idx:
+        p, seed = reqs[i]
+        _, st = tiny_mc_amd.run_raw(p, seed)
+        inter += st.interactions
+        photons += p.photons
+    wall = time.perf_counter() - t0
+    return {"requests": len(idx), "seconds": wall, "photons_per_s": photons / wall,
+            "interactions_per_s": inter / wall}
 
 
 def reference_case_leg(sampler, calls=20):
@@ -251,6 +322,24 @@ def reference_case_leg(sampler, calls=20):
         "kernel_ms_p50": float(sorted(kern)[len(kern) // 2]),
         "photons_per_s_per_call": p.photons / (sorted(total)[len(total) // 2] * 1e-3),
     }
+
+
+def rejection_leg(params, steps):
+    """The same walk with the reference's rejection sampler (photon.c:34-43)."""
+    import tiny_mc_amd
+
+    tiny_mc_amd.run_raw(params, 999, sampler=tiny_mc_amd.SAMPLER_REJECTION)
+    kernel_ms, inter = [], 0
+    t0 = time.perf_counter()
+    for i in range(steps):
+        _, st = tiny_mc_amd.run_raw(params, 500 + i, sampler=tiny_mc_amd.SAMPLER_REJECTION)
+        kernel_ms.append(st.kernel_ms)
+        inter += st.interactions
+    wall = time.perf_counter() - t0
+    k_s = sum(kernel_ms) / len(kernel_ms) * 1e-3
+    achieved = FLOP_PER_INTERACTION * inter / steps / k_s / 1e12
+    return {"photons_per_s": steps * params.photons / wall, "kernel_ms_avg": k_s * 1e3, "steps": steps,
+            "interactions_per_s": inter / wall, "roofline_achieved": achieved, "frac": achieved / PEAK_FP32_TFLOPS}
 
 
 def read_traffic():
@@ -283,7 +372,13 @@ def main():
     ap.add_argument("--no-reference-case", action="store_true",
                     help="skip the 32,768-photon call latency (keeps profiled runs to the 2^30-photon launches)")
     ap.add_argument("--serve-load", type=float, default=0.5)
-    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-seconds", type=float, default=12.0)
+    ap.add_argument("--saturation-seconds", type=float, default=8.0)
+    ap.add_argument("--no-rejection", action="store_true", help="skip the rejection-sampler figure")
+    ap.add_argument("--serve-max-chunk-scale", type=int, default=0,
+                    help="tmc_serve_options.max_chunk_scale of the served legs (0: the default)")
+    ap.add_argument("--serve-capacity", type=float, default=0.0,
+                    help="absorptions/s the serving loads are fractions of (0: the sweep's measured rate)")
     args = ap.parse_args()
     if args.rocpd_summary:
         return print(rocpd_summary(args.rocpd_summary), end="")
@@ -318,6 +413,7 @@ def main():
     avg_kernel_s = sum(kernel_ms) / len(kernel_ms) * 1e-3
     flop_per_launch = FLOP_PER_INTERACTION * interactions / args.steps
     achieved = flop_per_launch / avg_kernel_s / 1e12
+    kernel_flop = FLOP_PER_INTERACTION_ANALYTIC if args.sampler == "analytic" else FLOP_PER_INTERACTION
     result = {
         "metric": METRIC,
         "value": photons / elapsed,
@@ -354,11 +450,18 @@ def main():
             "traffic": read_traffic(),
             "kernel": "walk_f32",
             "flop_per_launch": flop_per_launch,
+            "flop_per_absorption": FLOP_PER_INTERACTION,
         },
+        "flop_per_absorption_kernel": {"analytic": FLOP_PER_INTERACTION_ANALYTIC, "rejection": FLOP_PER_INTERACTION},
+        "frac_kernel_flops": kernel_flop * interactions / args.steps / avg_kernel_s / 1e12 / PEAK_FP32_TFLOPS,
+        "weight_table_entries": st.weight_table,
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
     }
     if result["cpu_baseline"]:
         progress(f"cpu baseline: {result['cpu_baseline']['value']:.4g} photons/s")
+    if not args.no_rejection and args.sampler == "analytic":
+        result["rejection"] = rejection_leg(params, min(args.steps, 5))
+        progress(f"rejection sampler: {result['rejection']['photons_per_s']:.4g} photons/s")
     if not args.no_reference_case:
         result["reference_case"] = reference_case_leg(sampler)
     if not args.no_sweep:
@@ -367,8 +470,19 @@ def main():
                  f"{result['sweep']['speedup_vs_one_launch_per_set']:.3f}x one launch per set")
     if not args.no_serve:
         cap = result["sweep"]["interactions_per_s"] if "sweep" in result else interactions / elapsed
-        result["serve"] = serve_leg(cap, load=args.serve_load, seconds=args.serve_seconds)
+        cap = args.serve_capacity or cap
+        mcs = args.serve_max_chunk_scale
+        result["serve"], _, _ = serve_leg(cap, load=args.serve_load, seconds=args.serve_seconds, max_chunk_scale=mcs)
         progress(f"serve: p50 {result['serve']['p50_ms']:.3g} ms, p99 {result['serve']['p99_ms']:.3g} ms")
+        sat, reqs, _ = serve_leg(cap, load=1.0, seconds=args.saturation_seconds, seed=13, max_chunk_scale=mcs)
+        progress(f"serve at saturation: {sat['interactions_per_s']:.4g} absorptions/s, p50 {sat['p50_ms']:.3g} ms, "
+                 f"p99 {sat['p99_ms']:.3g} ms; now one launch per request")
+        plain = one_launch_per_request_leg(reqs)
+        sat["one_launch_per_request"] = plain
+        sat["served_vs_one_launch_per_request"] = sat["interactions_per_s"] / plain["interactions_per_s"]
+        sat["capacity_vs_batch"] = sat["interactions_per_s"] / cap
+        result["serve_saturation"] = sat
+        progress(f"serve: {sat['served_vs_one_launch_per_request']:.2f}x one launch per request")
     print(json.dumps(result))
 
 

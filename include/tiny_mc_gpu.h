@@ -296,8 +296,11 @@ int tmc_batch_run(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* 
 
 /* Picks block size, resident waves, tally copies and chunk work for this
  * batch by timing candidate launches on a reduced copy of it (every set's
- * photon budget divided by `sample_divisor`, at least 64 photons).  Writes
- * the fastest options to *best (starting from *base). */
+ * photon budget divided by `sample_divisor`, at least 64 photons; each
+ * candidate the best of three runs).  Starting from *base, a candidate
+ * replaces the current choice only when it is at least 3% faster, so the
+ * result stays *base unless the sample shows a clear gain.  Writes the
+ * choice to *best. */
 int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* base,
                        uint32_t sample_divisor, tmc_batch_options* best);
 

@@ -557,6 +557,7 @@ TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc
      * sample so that the sample sees the same chunk count per wave */
     const double scale = full_work > 0 ? sample_work / full_work : 1.0;
 
+    /* each candidate: the best of three runs after a warm-up run */
     auto time_of = [&](const tmc_batch_options& o, double mult) -> double {
         tmc_batch_options t = o;
         t.chunk_work = (uint64_t)std::max(4096.0, mult * (double)auto_cw * scale);
@@ -565,19 +566,24 @@ TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc
             return 1e300;
         tmc_batch_stats st;
         double best_ms = 1e300;
-        for (int rep = 0; rep < 2; ++rep) /* first run warms the code object */
+        for (int rep = 0; rep < 4; ++rep) /* first run warms the code object */
             if (tmc_batch_plan_run(plan, nullptr, &st) == TMC_OK && rep > 0)
-                best_ms = st.kernel_ms;
+                best_ms = std::min(best_ms, st.kernel_ms);
         tmc_batch_plan_destroy(plan);
         return best_ms;
     };
+    /* a candidate replaces the current options only when it is faster by
+     * kMargin on the sample: the sample's shorter queues favour shapes that
+     * do not carry over to the full batch, and the automatic options already
+     * keep nearly every lane busy */
+    constexpr double kMargin = 0.97;
 
     tmc_batch_options cur = b0;
     double cur_mult = 1.0;
     double cur_ms = time_of(cur, cur_mult);
     for (double m : { 0.25, 4.0 }) {
         const double ms = time_of(cur, m);
-        if (ms < cur_ms) {
+        if (ms < kMargin * cur_ms) {
             cur_ms = ms;
             cur_mult = m;
         }
@@ -586,7 +592,7 @@ TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc
         tmc_batch_options t = cur;
         t.copies = copies;
         const double ms = time_of(t, cur_mult);
-        if (ms < cur_ms) {
+        if (ms < kMargin * cur_ms) {
             cur_ms = ms;
             cur = t;
         }
@@ -595,7 +601,7 @@ TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc
         tmc_batch_options t = cur;
         t.block = block;
         const double ms = time_of(t, cur_mult);
-        if (ms < cur_ms) {
+        if (ms < kMargin * cur_ms) {
             cur_ms = ms;
             cur = t;
         }
@@ -604,7 +610,7 @@ TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc
         tmc_batch_options t = cur;
         t.waves_per_simd = wps;
         const double ms = time_of(t, cur_mult);
-        if (ms < cur_ms) {
+        if (ms < kMargin * cur_ms) {
             cur_ms = ms;
             cur = t;
         }
@@ -612,7 +618,8 @@ TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc
     if (cur_ms >= 1e300)
         return TMC_E_HIP;
     *best = cur;
-    best->chunk_work = (uint64_t)std::max(4096.0, cur_mult * (double)auto_cw);
+    /* unchanged chunk work stays automatic (the plan derives it per pass) */
+    best->chunk_work = cur_mult == 1.0 ? b0.chunk_work : (uint64_t)std::max(4096.0, cur_mult * (double)auto_cw);
     best->use_graph = base ? base->use_graph : 0;
     return TMC_OK;
 }

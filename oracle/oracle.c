@@ -445,8 +445,12 @@ typedef struct m32_params {
 } m32_params;
 
 /* Fixed-point images of one deposit: w * 2^33 and (w*w) * 2^43, truncated.
- * Exact (no rounding) for every w in [2^-10, 1], which covers every deposit
- * of the walk (the cutoff is 0.001 > 2^-10). */
+ * Exact (no rounding) for every w in [2^-10, 1].  With an albedo of 0.1 or
+ * more every deposit lies there (a roulette survivor re-enters at about
+ * 10 a times the weight it fell from, 0.001 or more); below 0.1 each
+ * survival shrinks the weight by about 10 a, and after enough survivals (two
+ * at albedo 0.05, about twenty at 0.09) the images of the deposits are
+ * truncated, as the kernel's are. */
 static inline uint64_t fx_heat(float w) { return (uint64_t)(w * 0x1p33f); }
 static inline uint64_t fx_heat2(float w) { return (uint64_t)((w * w) * 0x1p43f); }
 
@@ -755,63 +759,3 @@ ORC_EXPORT void orc_eval_math32(const float* in, uint64_t n, float* lg, float* s
     }
 }
 
-/* The reference loop of orc_restate (glibc rand(), FP32 transport in the
- * reference's operation order) with each FP32 deposit (1 - a) * w and its
- * square summed in long double instead of accumulated in float
- * (tiny_mc.c:26-27, photon.c:30-31): separates the reference's physics from
- * the rounding of its float tallies. */
-typedef struct exact_sink {
-    long double* heat;
-    long double* heat2;
-    float one_minus_albedo;
-} exact_sink;
-
-#define W_FN restate_exact_photon
-#define W_REAL float
-#define W_SRC glibc_src
-#define W_SINK exact_sink
-#define W_U_STEP(s, k) glibc_u()
-#define W_U_DIR(s, k, a, i) glibc_u()
-#define W_U_ROUL(s, k) glibc_u()
-#define W_LOG logf
-#define W_SQRT sqrtf
-#define W_SINCOS2PI(a, s, c) (s = 0.0f, c = 0.0f)
-#define W_DEPOSIT(sink, shell, wt)                                                                     \
-    do {                                                                                             \
-        const float d_ = (sink)->one_minus_albedo * (wt);                                            \
-        const float d2_ = (sink)->one_minus_albedo * (sink)->one_minus_albedo * (wt) * (wt);         \
-        (sink)->heat[shell] += d_;                                                                   \
-        (sink)->heat2[shell] += d2_;                                                                 \
-    } while (0)
-#include "orc_walk_body.h"
-#undef W_FN
-#undef W_REAL
-#undef W_SRC
-#undef W_SINK
-#undef W_U_STEP
-#undef W_U_DIR
-#undef W_U_ROUL
-#undef W_LOG
-#undef W_SQRT
-#undef W_SINCOS2PI
-#undef W_DEPOSIT
-
-ORC_EXPORT int orc_restate_exact(const orc_params* p, unsigned seed, uint32_t photons, double* heat, double* heat2)
-{
-    if (!orc_params_ok(p) || !heat || !heat2)
-        return -1;
-    long double* acc = calloc(2 * (size_t)p->shells, sizeof(long double));
-    if (!acc)
-        return -2;
-    exact_sink sink = { acc, acc + p->shells, 1.0f - restate_albedo(p) };
-    glibc_src src = { 0 };
-    srand(seed);
-    for (uint32_t i = 0; i < photons; ++i)
-        restate_exact_photon(&src, &sink, restate_albedo(p), restate_spm(p), p->shells, ORC_SAMPLER_REJECTION, 0);
-    for (uint32_t s = 0; s < p->shells; ++s) {
-        heat[s] = (double)acc[s];
-        heat2[s] = (double)acc[p->shells + s];
-    }
-    free(acc);
-    return 0;
-}

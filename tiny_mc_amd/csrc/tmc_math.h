@@ -209,10 +209,14 @@ __device__ __forceinline__ float div_tenth(float w)
 }
 
 /* Fixed-point images of one deposit: w * 2^33 and (w*w) * 2^43, truncated.
- * Exact for every w in [2^-10, 1]; the walk deposits only weights in
- * [0.001, 1] (weights below the 0.001 cutoff are rouletted before the next
- * deposit, photon.c:45-49), so the sums are the exact sums of the FP32
- * deposits. */
+ * Exact for every w in [2^-10, 1].  At an albedo of 0.1 or more every
+ * deposit lies there (a weight below the 0.001 cutoff is rouletted before
+ * the next deposit, photon.c:45-49, and a survivor re-enters at about 10 a
+ * times the weight it fell from), so the sums are the exact sums of the
+ * FP32 deposits.  Below 0.1 each survival shrinks the weight by about 10 a,
+ * and after enough survivals (two at albedo 0.05, about twenty at 0.09)
+ * deposits fall below 2^-10 and their images are truncated:
+ * deterministically, and the same in the host mirror. */
 #define TMC_HEAT_FRAC_BITS 33
 #define TMC_HEAT2_FRAC_BITS 43
 

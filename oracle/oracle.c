@@ -160,7 +160,7 @@ static void sb_printf(sbuf* b, const char* fmt, ...)
     va_list ap;
     va_start(ap, fmt);
     const size_t room = b->len < b->cap ? b->cap - b->len : 0;
-    const int n = vsnprintf(b->buf ? b->buf + b->len : NULL, room, fmt, ap);
+    const int n = vsnprintf(b->buf && room ? b->buf + b->len : NULL, room, fmt, ap);
     va_end(ap);
     if (n > 0)
         b->len += (size_t)n;

@@ -163,116 +163,6 @@ void tmc::serving_kernel_left()
         fn();
 }
 
-/* Parameter checks.  The reference has none (params.h:14 only warns that
- * MU_A must be non-zero: with mu_a = 0 the weight never reaches the cutoff
- * and photon() never returns). */
-int tmc::check_params(const tmc_params* p)
-{
-    if (!p)
-        return TMC_E_INVALID;
-    if (!std::isfinite(p->mu_a) || !std::isfinite(p->mu_s) || !std::isfinite(p->microns_per_shell))
-        return TMC_E_INVALID;
-    if (!(p->mu_a > 0.0f) || !(p->mu_s >= 0.0f) || !(p->microns_per_shell > 0.0f))
-        return TMC_E_INVALID;
-    if (p->shells < 1 || p->shells > TMC_MAX_SHELLS)
-        return TMC_E_INVALID;
-    /* binary32 albedo must stay below 1, or the weight never decays */
-    const volatile float sum = p->mu_s + p->mu_a;
-    const volatile float albedo = p->mu_s / sum;
-    if (!(albedo < 1.0f))
-        return TMC_E_INVALID;
-    const double spm = 1e4 / (double)p->microns_per_shell / (double)sum;
-    if (!std::isfinite((float)spm) || !((float)spm > 0.0f))
-        return TMC_E_INVALID;
-    return TMC_OK;
-}
-
-/* photon.c:8: albedo evaluated in float. */
-float tmc::derived_albedo(const tmc_params* p)
-{
-    const volatile float sum = p->mu_s + p->mu_a;
-    return p->mu_s / sum;
-}
-
-/* photon.c:9: 1e4 is a double literal, so the quotient is formed in double
- * from the float sum mu_a + mu_s, then stored as float. */
-float tmc::derived_spm(const tmc_params* p)
-{
-    const volatile float sum = p->mu_a + p->mu_s;
-    return (float)(1e4 / (double)p->microns_per_shell / (double)sum);
-}
-
-/* The weight after k absorptions is a deterministic binary32 sequence
- * (photon.c:32, 45-49): it falls below 0.001 after k1 steps, then after k2
- * steps of each later cycle (the first of them; later cycles can differ by
- * a step).  False if the first cycle is longer than `limit` steps. */
-bool tmc::weight_cadence(const tmc_params* p, uint32_t limit, uint32_t* k1_out, uint32_t* k2_out)
-{
-    const float a = derived_albedo(p);
-    float w = 1.0f;
-    uint32_t k1 = 0;
-    while (w >= 0.001f && k1 < limit) {
-        w = w * a;
-        ++k1;
-    }
-    if (!(w < 0.001f))
-        return false;
-    /* later cycles start from the cutoff weight / 0.1 */
-    float v = w / 0.1f;
-    uint32_t k2 = 0;
-    while (v >= 0.001f && k2 < limit) {
-        v = v * a;
-        ++k2;
-    }
-    *k1_out = k1;
-    *k2_out = k2 ? k2 : 1;
-    return true;
-}
-
-/* Mean absorptions k1 + k2 p / (1 - p) with p the survival probability
- * 1677721 / 2^24 of the (0,1] uniform. */
-double tmc::expected_interactions(const tmc_params* p, uint64_t kcap)
-{
-    const double pr = 1677721.0 / 16777216.0;
-    uint32_t k1 = 0, k2 = 0;
-    double c1, c2;
-    if (weight_cadence(p, 1u << 20, &k1, &k2)) {
-        c1 = (double)k1;
-        c2 = (double)k2;
-    } else {
-        /* albedo so close to 1 that the cadence is long: continuous estimate */
-        const double a = (double)derived_albedo(p);
-        c1 = std::log(0.001) / std::log(a);
-        c2 = std::log(0.1) / std::log(a);
-    }
-    double mean = c1 + c2 * pr / (1.0 - pr);
-    if (kcap && mean > (double)kcap)
-        mean = (double)kcap;
-    return mean < 1.0 ? 1.0 : mean;
-}
-
-/* Entries of the binary32 walk's weight table (0: the walk multiplies).
- * The table is built per workgroup when the launch starts, so small runs
- * multiply; it must span the first cycle and eight more (a photon survives
- * eight roulettes with probability 1e-8), or the lanes past its end would
- * run the arithmetic beside it on most steps.  Automatic for the analytic
- * sampler only: the rejection sampler's retry iterations read the entry
- * for nothing (measured 1.4% slower with the table at default physics). */
-uint32_t tmc::weight_table_entries(const tmc_params* p, uint32_t precision, uint32_t sampler, uint64_t photons,
-                                   uint32_t request)
-{
-    if (precision != TMC_PRECISION_FP32 || request == TMC_WEIGHT_TABLE_OFF)
-        return 0;
-    if (request)
-        return std::min<uint32_t>(request, TMC_WALK_TABLE_MAX);
-    if (sampler != TMC_SAMPLER_ANALYTIC || photons < (1ull << 20))
-        return 0;
-    uint32_t k1 = 0, k2 = 0;
-    if (!weight_cadence(p, TMC_WALK_TABLE_MAX, &k1, &k2) || (uint64_t)k1 + 8ull * k2 > TMC_WALK_TABLE_MAX)
-        return 0;
-    return (uint32_t)std::min<uint64_t>(TMC_WALK_TABLE_MAX, (uint64_t)k1 + 24ull * k2);
-}
-
 namespace {
 
 double u128_to_double(uint64_t lo, uint64_t hi)

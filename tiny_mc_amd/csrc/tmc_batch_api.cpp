@@ -179,7 +179,7 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
 
     /* set table */
     std::vector<tmc::BatchSet> hs(nsets);
-    std::vector<double> L(nsets);
+    std::vector<tmc::PlanSet> ps(nsets);
     uint64_t off = 0;
     double work = 0.0;
     uint32_t shells_max = 1;
@@ -200,131 +200,50 @@ int plan_create_locked(tmc::Context* c, const tmc_set* sets, uint32_t nsets, con
         d.kcap = (uint32_t)kcap;
         off += 4 * (uint64_t)q.shells;
         shells_max = std::max(shells_max, q.shells);
-        L[i] = tmc::expected_interactions(&q, kcap);
-        work += L[i] * (double)q.photons;
+        ps[i] = { q.shells, q.photons, tmc::expected_interactions(&q, kcap) };
+        work += ps[i].L * (double)q.photons;
         p->photons += q.photons;
     }
     p->tally_words = off;
     p->shells_max = shells_max;
     p->expected = work;
 
-    /* LDS size classes: the distinct shell counts, ascending, grouped by the
-     * launch shape they allow */
-    std::vector<uint32_t> svals;
-    for (uint32_t i = 0; i < nsets; ++i)
-        svals.push_back(sets[i].params.shells);
-    std::sort(svals.begin(), svals.end());
-    svals.erase(std::unique(svals.begin(), svals.end()), svals.end());
-    std::vector<double> swork(svals.size(), 0.0);
-    for (uint32_t i = 0; i < nsets; ++i) {
-        const size_t k = std::lower_bound(svals.begin(), svals.end(), sets[i].params.shells) - svals.begin();
-        swork[k] += L[i] * (double)sets[i].params.photons;
-    }
-    struct Group {
-        uint32_t lo, hi;
-        PassPlan shape;
-    };
-    std::vector<Group> groups;
-    for (size_t k = 0; k < svals.size(); ++k) {
+    /* launch passes and chunk queues */
+    const uint32_t cus = (uint32_t)c->compute_units;
+    const tmc::ShapeFn shape_of = [&](uint32_t shells, tmc::PlanShape* shape) {
         PassPlan s;
         bool fits = false;
-        if (launch_shape(p, o, (uint32_t)c->compute_units, svals[k], &s, &fits) != hipSuccess) {
-            plan_free(p);
-            return TMC_E_HIP;
-        }
-        if (!fits) {
-            plan_free(p);
-            return TMC_E_INVALID; /* LDS slots of this many shells do not fit */
-        }
-        s.work = swork[k];
-        if (!groups.empty() && groups.back().shape.grid == s.grid && groups.back().shape.block == s.block) {
-            groups.back().hi = svals[k];
-            groups.back().shape.shells_max = svals[k];
-            groups.back().shape.work += s.work;
-        } else {
-            groups.push_back({ svals[k], svals[k], s });
-        }
-    }
-    /* a class with under a tenth of the work joins its neighbour with more
-     * shells (whose slots hold both), and there are at most
-     * TMC_BATCH_MAX_PASSES passes */
-    while (groups.size() > 1) {
-        size_t m = 0;
-        for (size_t k = 1; k < groups.size(); ++k)
-            if (groups[k].shape.work < groups[m].shape.work)
-                m = k;
-        if (groups[m].shape.work >= 0.1 * work && groups.size() <= TMC_BATCH_MAX_PASSES)
-            break;
-        const size_t lo_i = m + 1 < groups.size() ? m : m - 1, hi_i = lo_i + 1;
-        Group g = groups[hi_i];
-        g.lo = groups[lo_i].lo;
-        g.shape.work += groups[lo_i].shape.work;
-        groups[lo_i] = g;
-        groups.erase(groups.begin() + (ptrdiff_t)hi_i);
-    }
-
-    /* chunk queues, one per pass */
-    struct Item {
-        tmc::BatchChunk ch;
-        double dur;
+        if (launch_shape(p, o, cus, shells, &s, &fits) != hipSuccess)
+            return (int)TMC_E_HIP;
+        if (!fits)
+            return (int)TMC_E_INVALID; /* LDS slots of this many shells do not fit */
+        *shape = { s.grid, s.block, s.wps };
+        return (int)TMC_OK;
     };
-    std::vector<std::vector<Item>> items(groups.size());
-    for (size_t g = 0; g < groups.size(); ++g) {
-        PassPlan& q = groups[g].shape;
-        const double waves = (double)q.grid * (q.block / 64u);
-        q.chunk_work = o->chunk_work ? o->chunk_work : (uint64_t)std::max(65536.0, q.work / (16.0 * waves));
+    std::vector<tmc::PlanPass> passes;
+    std::vector<tmc::PlanChunk> pc;
+    const int prc = tmc::plan_passes(ps, o->chunk_work, TMC_BATCH_MAX_PASSES, shape_of, &passes, &pc);
+    if (prc != TMC_OK) {
+        plan_free(p);
+        return prc;
     }
-    for (uint32_t i = 0; i < nsets; ++i) {
-        const uint64_t N = sets[i].params.photons;
-        if (!N)
-            continue;
-        size_t g = 0;
-        while (sets[i].params.shells > groups[g].hi)
-            ++g;
-        const uint64_t cw = groups[g].shape.chunk_work;
-        double per_lane = std::ceil((double)cw / (64.0 * L[i]));
-        per_lane = std::min(std::max(per_lane, 1.0), (double)(1u << 24));
-        uint64_t C = 64u * (uint64_t)per_lane;
-        /* keep the queue below 2^24 chunks */
-        const uint64_t min_c = (N >> 24) + 1;
-        if (C < min_c)
-            C = (min_c + 63) / 64 * 64;
-        if (C > (1ull << 31))
-            C = 1ull << 31;
-        for (uint64_t first = 0; first < N; first += C) {
-            const uint64_t cnt = std::min(C, N - first);
-            items[g].push_back({ { i, (uint32_t)cnt, first }, (double)((cnt + 63) / 64) * L[i] });
-        }
-    }
-    for (auto& v : items)
-        std::stable_sort(v.begin(), v.end(), [](const Item& x, const Item& y) { return x.dur > y.dur; });
-    /* the pass with the longest chunk first, so that the launch sequence ends
-     * on short chunks */
-    std::vector<size_t> order(groups.size());
-    for (size_t g = 0; g < order.size(); ++g)
-        order[g] = g;
-    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-        const double dx = items[x].empty() ? 0.0 : items[x][0].dur, dy = items[y].empty() ? 0.0 : items[y][0].dur;
-        return dx > dy;
-    });
-    std::vector<tmc::BatchChunk> hc;
-    std::vector<uint64_t> first_chunk(groups.size());
-    p->npasses = (uint32_t)groups.size();
-    for (size_t j = 0; j < order.size(); ++j) {
-        const size_t g = order[j];
-        first_chunk[j] = hc.size();
-        for (const Item& it : items[g])
-            hc.push_back(it.ch);
-        p->pass[j] = groups[g].shape;
-        p->pass[j].nchunks = items[g].size();
+    std::vector<tmc::BatchChunk> hc(pc.size());
+    for (size_t k = 0; k < pc.size(); ++k)
+        hc[k] = { pc[k].set, pc[k].count, pc[k].first };
+    p->npasses = (uint32_t)passes.size();
+    std::vector<uint64_t> first_chunk(passes.size());
+    for (size_t j = 0; j < passes.size(); ++j) {
+        PassPlan& q = p->pass[j];
+        q.grid = passes[j].shape.grid;
+        q.block = passes[j].shape.block;
+        q.wps = passes[j].shape.wps;
+        q.shells_max = passes[j].hi;
+        q.work = passes[j].work;
+        q.chunk_work = passes[j].chunk_work;
+        q.nchunks = passes[j].nchunks;
+        first_chunk[j] = passes[j].first_chunk;
     }
     p->nchunks = hc.size();
-    p->shells_max = shells_max;
-    for (uint32_t j = 0; j < p->npasses; ++j)
-        if (p->pass[j].nchunks > UINT32_MAX) {
-            plan_free(p);
-            return TMC_E_RANGE; /* a pass's queue length is a 32-bit kernel argument */
-        }
     uint32_t main_pass = 0;
     for (uint32_t j = 1; j < p->npasses; ++j)
         if (p->pass[j].work > p->pass[main_pass].work)

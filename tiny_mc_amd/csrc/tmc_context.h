@@ -9,6 +9,7 @@
 #include <mutex>
 
 #include "../../include/tiny_mc_gpu.h"
+#include "tmc_plan.h"
 
 namespace tmc {
 
@@ -47,38 +48,6 @@ int run_walk(Context* c, const tmc_params* p, uint64_t seed, const tmc_options* 
 /* The static scratch of tiny_mc_gpu_run_noalloc on `device` (current):
  * looks its address up once per device.  TMC_OK or a negative status. */
 int scratch_setup(int device);
-
-/* Parameter validation shared by every entry point (TMC_OK or
- * TMC_E_INVALID). */
-int check_params(const tmc_params* p);
-
-/* Whether the photon index range [first, first + photons) lies below 2^63:
- * then no photon index, chunk end or claim counter (which runs past the end
- * of the range by at most one chunk per wave) wraps around 64 bits. */
-inline bool photon_range_ok(uint64_t first, uint64_t photons)
-{
-    const uint64_t limit = 1ull << 63;
-    return first <= limit && photons <= limit - first;
-}
-
-/* photon.c:8: albedo = mu_s / (mu_s + mu_a) in binary32. */
-float derived_albedo(const tmc_params* p);
-
-/* photon.c:9: shells per mean free path through a double quotient. */
-float derived_spm(const tmc_params* p);
-
-/* Expected absorptions per photon of the FP32 weight cadence with the
- * roulette of photon.c:45-49, capped at kcap (scheduling estimate). */
-double expected_interactions(const tmc_params* p, uint64_t kcap);
-
-/* The binary32 weight cadence: absorptions k1 of the first cycle and k2 of
- * the second (false if the first is longer than `limit`). */
-bool weight_cadence(const tmc_params* p, uint32_t limit, uint32_t* k1, uint32_t* k2);
-
-/* Entries of the single run's weight table for these parameters and the
- * caller's request (tmc_options.weight_table; 0: none). */
-uint32_t weight_table_entries(const tmc_params* p, uint32_t precision, uint32_t sampler, uint64_t photons,
-                              uint32_t request);
 
 /* Releases of device resources while a serving kernel may be resident.
  * hipFree and hipHostFree synchronise the whole device, so under load they

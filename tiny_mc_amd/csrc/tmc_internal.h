@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tmc_plan.h"
+
 namespace tmc {
 
 /* Everything one launch of the walk kernel needs, passed by value. */
@@ -32,9 +34,6 @@ struct WalkArgs {
     float spm_w; /* spm_d rounded to float */
     uint32_t pad2;
 };
-
-/* Largest weight table of the binary32 walk (entries of 16 bytes in LDS). */
-#define TMC_WALK_TABLE_MAX 1024u
 
 /* Launches the persistent walk kernel of `precision` (TMC_PRECISION_*) on
  * `stream`.  LDS: 16 bytes per shell plus the overflow shell's lane copies,

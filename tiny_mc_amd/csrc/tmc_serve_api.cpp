@@ -83,42 +83,6 @@ struct Request {
     std::vector<uint64_t> result;
 };
 
-/* First-fit allocator of the tally arena (units: 64-bit words). */
-struct Arena {
-    std::map<uint64_t, uint64_t> free_; /* offset -> size */
-    void init(uint64_t words) { free_ = { { 0, words } }; }
-    bool alloc(uint64_t n, uint64_t* off)
-    {
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second >= n) {
-                *off = it->first;
-                const uint64_t rest = it->second - n, at = it->first + n;
-                free_.erase(it);
-                if (rest)
-                    free_[at] = rest;
-                return true;
-            }
-        }
-        return false;
-    }
-    void release(uint64_t off, uint64_t n)
-    {
-        auto it = free_.emplace(off, n).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-};
-
 /* Workgroup slots the serving kernel leaves free beyond reserve_per_cu per
  * compute unit: room for the runtime's one-workgroup kernels.  A kernel of
  * many workgroups needs more (measured on MI355X, 256 CUs, beside a resident
@@ -175,7 +139,7 @@ struct tmc_server {
     std::map<uint64_t, int> state; /* id -> kQueuedOrActive, kFinished or the status of a cancelled request */
     std::vector<uint32_t> free_slots;
     std::vector<uint32_t> slot_gen;
-    Arena arena;
+    tmc::Arena arena;
     double unpublished_work = 0; /* expected absorptions of the photons not yet published */
     uint64_t published = 0; /* ring indices written (the tail) */
     uint64_t claimed = 0; /* highest claim the kernel reported */
@@ -221,17 +185,6 @@ void server_free(tmc_server* s)
     for (auto& kv : s->finished)
         delete kv.second;
     delete s;
-}
-
-/* Photons of the next chunk of a request with `left` unpublished photons:
- * about `work` absorptions (so a chunk lasts about work / 64 steps of a
- * wave, which bounds how long a new request waits for waves), between 1
- * and `max_per_lane` photons per lane, never more than `left`. */
-uint64_t chunk_photons(double work, double L, uint64_t max_per_lane, uint64_t left)
-{
-    double per_lane = std::floor(work / (64.0 * L));
-    per_lane = std::min(std::max(per_lane, 1.0), (double)max_per_lane);
-    return std::max<uint64_t>(1, std::min<uint64_t>(left, 64u * (uint64_t)per_lane));
 }
 
 void store_release(uint64_t* p, uint64_t v) { __atomic_store_n(p, v, __ATOMIC_RELEASE); }
@@ -415,7 +368,7 @@ void serve_step(tmc_server* s)
         for (Request* r : order) {
             const uint64_t N = r->set.params.photons;
             while (r->next < N && unclaimed() < backlog_max) {
-                const uint64_t cnt = chunk_photons(work, r->L, max_per_lane, N - r->next);
+                const uint64_t cnt = tmc::chunk_photons(work, r->L, max_per_lane, N - r->next);
                 if (!publish_entry(s, r->slot | (r->gen << TMC_SERVE_SLOT_BITS), (uint32_t)cnt, r->next)) {
                     ring_full = true;
                     break;

@@ -374,6 +374,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.5)
     ap.add_argument("--serve-seconds", type=float, default=12.0)
     ap.add_argument("--saturation-seconds", type=float, default=8.0)
+    ap.add_argument("--no-saturation", action="store_true",
+                    help="skip the served run at saturation and its one-launch-per-request comparison")
     ap.add_argument("--no-rejection", action="store_true", help="skip the rejection-sampler figure")
     ap.add_argument("--serve-max-chunk-scale", type=int, default=0,
                     help="tmc_serve_options.max_chunk_scale of the served legs (0: the default)")
@@ -474,6 +476,7 @@ def main():
         mcs = args.serve_max_chunk_scale
         result["serve"], _, _ = serve_leg(cap, load=args.serve_load, seconds=args.serve_seconds, max_chunk_scale=mcs)
         progress(f"serve: p50 {result['serve']['p50_ms']:.3g} ms, p99 {result['serve']['p99_ms']:.3g} ms")
+    if not args.no_serve and not args.no_saturation:
         sat, reqs, _ = serve_leg(cap, load=1.0, seconds=args.saturation_seconds, seed=13, max_chunk_scale=mcs)
         progress(f"serve at saturation: {sat['interactions_per_s']:.4g} absorptions/s, p50 {sat['p50_ms']:.3g} ms, "
                  f"p99 {sat['p99_ms']:.3g} ms; now one launch per request")

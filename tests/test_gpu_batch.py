@@ -95,7 +95,12 @@ def test_graph_replays_identical(tmc):
 def test_autotune_returns_working_options(tmc):
     sets = small_sweep(tmc, n=32, seed=33)
     best = tmc.batch_autotune(sets, sample_divisor=4)
-    assert best.chunk_work > 0
+    # each field is either kept automatic (0, as in the default base) or a
+    # candidate the tuner timed
+    assert best.block in (0, 64, 256, 512)
+    assert best.copies in (0, 1, 2)
+    assert best.waves_per_simd in (0, 4, 6)
+    assert best.chunk_work >= 0
     got, st = tmc.batch_run(sets, best)
     base, _ = tmc.batch_run(sets)
     np.testing.assert_array_equal(got, base)

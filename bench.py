@@ -303,6 +303,21 @@ def one_launch_per_request_leg(reqs, count=1500):
             "interactions_per_s": inter / wall}
 
 
+def same_work_batch_leg(reqs):
+    """The served stream's requests as one batched plan (default options),
+    timed like the sweep's plan runs: the batch kernel's rate on the very
+    work the server walked, for the served/batch capacity ratio."""
+    import tiny_mc_amd
+
+    with tiny_mc_amd.BatchPlan(reqs, tiny_mc_amd.BatchOptions(use_graph=True)) as plan:
+        t = time.perf_counter()
+        _, st = plan.run()
+        wall = time.perf_counter() - t
+    photons = sum(p.photons for p, _ in reqs)
+    return {"requests": len(reqs), "seconds": wall, "photons_per_s": photons / wall,
+            "interactions_per_s": st.interactions / wall, "lane_utilisation": st.lane_utilisation}
+
+
 def reference_case_leg(sampler, calls=20):
     """The reference's own configuration (params.h defaults: 32,768 photons,
     101 shells): too small to fill the GPU, so the figure is the latency of
@@ -484,8 +499,13 @@ def main():
         sat["one_launch_per_request"] = plain
         sat["served_vs_one_launch_per_request"] = sat["interactions_per_s"] / plain["interactions_per_s"]
         sat["capacity_vs_batch"] = sat["interactions_per_s"] / cap
+        progress("serve: the same requests as one batch")
+        same = same_work_batch_leg(reqs)
+        sat["same_work_batch"] = same
+        sat["capacity_vs_same_work_batch"] = sat["interactions_per_s"] / same["interactions_per_s"]
         result["serve_saturation"] = sat
-        progress(f"serve: {sat['served_vs_one_launch_per_request']:.2f}x one launch per request")
+        progress(f"serve: {sat['served_vs_one_launch_per_request']:.2f}x one launch per request, "
+                 f"{sat['capacity_vs_same_work_batch']:.3f} of the same work batched")
     print(json.dumps(result))
 
 

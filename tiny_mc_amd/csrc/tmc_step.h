@@ -202,12 +202,12 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
                                      const WeightTable& wt = WeightTable{ nullptr, 0u, 0.0f })
 {
     const bool fresh = absorbs<SAMPLER>(p);
-    /* the table entry first: its load overlaps the generator */
+    /* the table entry first: its load overlaps the generator (where exactly
+     * is left to the compiler: pinning the load ahead of the generator with
+     * a scheduling barrier measured 1.3% slower) */
     uint4 e = { 0u, 0u, 0u, 0u };
-    if (TABLE) {
-        e = wt.entry[p.k < wt.k_end ? p.k : wt.k_end - 1u];
-        __builtin_amdgcn_sched_barrier(0); /* keep the load ahead of the generator */
-    }
+    if (TABLE)
+        e = wt.entry[__builtin_elementwise_min(p.k, wt.k_end - 1u)];
     tmc_u32x4 r;
     if (SAMPLER == 1u) /* block (k, 0) for an absorption, (k, b) for attempts 2b and 2b + 1 */
         r = tmc_philox4x32_10_blockb(p.k, (p.att & 0xFFu) ? ((p.att & 0xFFu) + 1u) >> 1 : 0u, p.pp,

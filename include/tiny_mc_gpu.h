@@ -132,9 +132,12 @@ int tiny_mc_gpu_run(const tmc_params* p, uint64_t seed, double* heat, double* he
  * tally has 4 * p->shells words; words [2s, 2s+1] are the (low, high) halves
  * of the 128-bit sum of w * 2^TMC_HEAT_FRAC_BITS over deposits in shell s and
  * words [2S + 2s, 2S + 2s + 1] those of (w*w) * 2^TMC_HEAT2_FRAC_BITS
- * (w in binary32: both images are exact; TMC_PRECISION_FP64: rounded to
- * nearest).  Tallies of disjoint photon ranges of one stream add as
- * integers.  stats may be NULL. */
+ * (w in binary32: both images are exact for w >= 2^-10, which every deposit
+ * is at albedos of 0.1 or more; below that, deposits of photons that
+ * survived several roulettes can be smaller and their images are truncated,
+ * deterministically; TMC_PRECISION_FP64: rounded to nearest).  Tallies of
+ * disjoint photon ranges of one stream add as integers.  stats may be
+ * NULL. */
 int tiny_mc_gpu_run_ex(const tmc_params* p, uint64_t seed, const tmc_options* opt, uint64_t* tally,
                        tmc_run_stats* stats);
 

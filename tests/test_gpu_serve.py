@@ -306,3 +306,30 @@ def test_tiny_ring_wraps_many_times(tmc):
     assert st["chunks_published"] >= 20 * 1024 > 300 * 64
     for (p, s), (t, _) in zip(reqs, got):
         np.testing.assert_array_equal(t, alone(tmc, p, s))
+
+
+def test_served_results_within_tolerance_of_fp64_oracle(tmc, orc):
+    """Per-request accuracy at the north star's bar: each served result's
+    heat, shell by shell, within max(3 sigma, 1e-5 relative) of the FP64
+    oracle on the same stream, for requests of four physics served together
+    (sigma: the larger of the heat2 estimate and the standard error over 8
+    photon batches of the oracle)."""
+    cases = [(dict(), 1 << 18, 11),
+             (dict(mu_a=10.0, mu_s=10.0, shells=51, microns_per_shell=100.0), 1 << 18, 12),
+             (dict(mu_a=0.5, mu_s=50.0, shells=201, microns_per_shell=20.0), 1 << 15, 13),
+             (dict(mu_a=1.0, mu_s=60.0, shells=401, microns_per_shell=10.0), 1 << 16, 14)]
+    with tmc.Server() as srv:
+        ids = [srv.submit(tmc.Params(photons=N, **phys), seed) for phys, N, seed in cases]
+        got = [wait(srv, i) for i in ids]
+    for (phys, N, seed), (tally, _) in zip(cases, got):
+        p = tmc.Params(photons=N, **phys)
+        heat, _ = tmc.tally_to_heat(p, tally)
+        B = 8
+        parts = [orc.walk64(orc.params(**phys), seed, N // B, first=b * (N // B)) for b in range(B)]
+        batches = np.array([h for h, _, _ in parts])
+        oh = batches.sum(axis=0)
+        oh2 = np.sum([h2 for _, h2, _ in parts], axis=0)
+        sigma = np.maximum(np.sqrt(np.maximum(oh2 - oh * oh / N, 0.0)), np.sqrt(B) * batches.std(axis=0, ddof=1))
+        tol = np.maximum(3.0 * sigma, 1e-5 * np.abs(oh))
+        bad = np.nonzero(np.abs(heat - oh) > tol)[0]
+        assert bad.size == 0, (phys, bad, heat[bad], oh[bad], tol[bad])

@@ -127,23 +127,33 @@ __device__ __forceinline__ unsigned long long take_sys(unsigned long long* p)
     return __hip_atomic_exchange(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-/* Loads set `index` into the wave-uniform state.  Batch tables are device
- * memory written before the launch; serving tables are written by the host
- * while the kernel runs, so lane 0 reads them with system-scope loads and
- * broadcasts the words. */
-/* Whether walk_multi keeps a set's Philox round keys in scalar registers:
- * the batch kernel with the analytic sampler has the registers to spare;
- * the serving kernel and the rejection sampler (more live state) would spill
- * them, so there the step forms the keys from the seed. */
-template <uint32_t SAMPLER, bool SERVE>
-__host__ __device__ constexpr bool batch_keys()
+/* The kernel's arguments where a rare path (a claim, a flush, a set switch)
+ * uses them.  Serving: read through the argument block's pointer passed
+ * through an empty asm, so each rare path loads the fields it needs (scalar
+ * loads, cached) and no load is hoisted out of it: the serving kernel's loop
+ * then holds no argument in scalar registers, which leaves room for the
+ * set's Philox round keys.  Batch: the arguments as passed. */
+template <bool SERVE>
+__device__ __forceinline__ const BatchArgs& args_at_use(const BatchArgs& a)
 {
-    return !SERVE && SAMPLER == 0u;
+    if (!SERVE)
+        return a;
+    typedef const __attribute__((address_space(4))) BatchArgs* ConstArgs;
+    ConstArgs p = (ConstArgs)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const BatchArgs*)p;
 }
 
+/* Loads set `index` into the wave-uniform state, with the set's Philox
+ * round keys formed once and pinned to scalar registers (formed inside the
+ * step instead, they cost 18 scalar additions per absorption).  Batch
+ * tables are device memory written before the launch; serving tables are
+ * written by the host while the kernel runs, so lane 0 reads them with
+ * system-scope loads and broadcasts the words. */
 template <uint32_t SAMPLER, bool SERVE>
-__device__ __forceinline__ void load_set(SetState& st, const BatchArgs& a, uint32_t index, uint32_t lane)
+__device__ __forceinline__ void load_set(SetState& st, const BatchArgs& args, uint32_t index, uint32_t lane)
 {
+    const BatchArgs& a = args_at_use<SERVE>(args);
     /* words: seed, first_photon, tally_offset, (albedo | spm << 32),
      * (shell_max | shells << 32), (kcap | pad << 32), photons, token */
     constexpr int NW = SERVE ? 8 : 6;
@@ -171,7 +181,7 @@ __device__ __forceinline__ void load_set(SetState& st, const BatchArgs& a, uint3
     st.phys.spm = __uint_as_float((uint32_t)(w[3] >> 32));
     st.phys.shell_max = __uint_as_float((uint32_t)w[4]);
     st.shells = (uint32_t)(w[4] >> 32);
-    set_derived<batch_keys<SAMPLER, SERVE>()>(st.phys);
+    set_derived<true>(st.phys);
     st.kcap = (uint32_t)w[5];
     st.done = a.done + 2 * (uint64_t)index;
     st.photons = w[6];
@@ -196,9 +206,10 @@ enum ClaimResult { CLAIM_GOT, CLAIM_EMPTY, CLAIM_DONE };
  * every published entry was taken, so an entry at the ticket can only come
  * after the launch has ended. */
 template <bool SERVE>
-__device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t lane, uint32_t group, uint64_t& ticket,
-                                                 BatchChunk& ch)
+__device__ __forceinline__ ClaimResult try_claim(const BatchArgs& args, uint32_t lane, uint32_t group,
+                                                 uint64_t& ticket, BatchChunk& ch)
 {
+    const BatchArgs& a = args_at_use<SERVE>(args);
     if (!SERVE) {
         unsigned long long c = 0;
         if (lane == 0)
@@ -286,8 +297,9 @@ __device__ __forceinline__ ClaimResult try_claim(const BatchArgs& a, uint32_t la
  * leaves the region clear for the slot's next request.  The sums go to
  * the host mirror, then, once those writes have completed, the token goes
  * to the slot's flag. */
-__device__ __forceinline__ void finish_request(const BatchArgs& a, const SetState& st, uint32_t lane)
+__device__ __forceinline__ void finish_request(const BatchArgs& args, const SetState& st, uint32_t lane)
 {
+    const BatchArgs& a = args_at_use<true>(args);
     const uint32_t words = 4 * st.shells;
     for (uint32_t i = lane; i < words; i += 64)
         store_sys((uint64_t*)a.out + st.tally_offset + i, take_sys(st.tally + i));
@@ -332,10 +344,11 @@ __device__ __forceinline__ void publish_skip(const BatchArgs& a, const SetState&
  * update to complete before it adds to the counters, so the wave whose
  * count completes the request finds final sums. */
 template <bool SERVE>
-__device__ __forceinline__ void flush_and_publish(unsigned long long* slot, const BatchArgs& a, const SetState& st,
+__device__ __forceinline__ void flush_and_publish(unsigned long long* slot, const BatchArgs& args, const SetState& st,
                                                   uint32_t lane, uint32_t& completed, uint32_t& absorbed,
                                                   uint32_t& steps)
 {
+    const BatchArgs& a = args_at_use<SERVE>(args);
     flush_slot<SERVE>(slot, a.shells_max, a.copies, st.shells, st.tally, lane);
     if (!completed && !absorbed)
         return;
@@ -508,7 +521,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             /* the loop has one back-edge (no continue): with several, the
              * compiler copied the photon state in and out of the lane-masked
              * step every iteration */
-            if (ph.weight >= 0 && !step<SAMPLER, MATH, batch_keys<SAMPLER, SERVE>()>(ph, st.phys, lt))
+            if (ph.weight >= 0 && !step<SAMPLER, MATH>(ph, st.phys, lt))
                 ph.weight = -1;
             /* photons that reached the absorption cap stop; checked only while
              * some photon of the wave can reach it before the next flush */
@@ -516,7 +529,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 const uint64_t capped = __ballot(ph.k >= st.kcap) & __ballot(ph.weight >= 0);
                 if (capped != 0) {
                     if (lane == 0)
-                        atomicAdd(a.counters + 2, (unsigned long long)__popcll(capped));
+                        atomicAdd(args_at_use<SERVE>(a).counters + 2, (unsigned long long)__popcll(capped));
                     if (ph.k >= st.kcap)
                         ph.weight = -1;
                 }

@@ -49,7 +49,7 @@ __device__ __forceinline__ void set_derived(Physics& ph)
         /* where the keys change inside the walk loop (a new parameter set),
          * the compiler recomputes each round key inside the step (18 scalar
          * additions per absorption); the empty asm keeps them in scalar
-         * registers instead, for kernels with the registers to spare */
+         * registers instead */
 #pragma unroll
         for (int r = 1; r < 10; ++r) {
             ph.keys.k0[r] = __builtin_amdgcn_readfirstlane(ph.keys.k0[r]);
@@ -162,13 +162,6 @@ __device__ __forceinline__ bool cap_within(const PhotonT& ph, uint32_t kcap, uin
  * i.e. r >= 1677721 * 2^8: one integer compare on the Philox word. */
 #define TMC_ROULETTE_KILL_R 0x19999900u
 
-/* Walks one absorption event of p, adding its deposit to the fixed-point
- * shell sums in LDS.  Returns false when the roulette kills the photon.
- * MATH (TMC_MATH_*) selects how -ln u, sin/cos(2 pi u') and the analytic
- * sampler's sin(theta) are evaluated; everything else is the same
- * arithmetic in both modes.  STORED_KEYS takes the Philox round keys from
- * ph.keys; otherwise they are formed from the seed inside the step (fewer
- * live registers, for kernels short of scalar registers). */
 /* Whether the walk's next iteration of p is an absorption (the rejection
  * sampler's retries of a direction are iterations without one). */
 template <uint32_t SAMPLER>
@@ -183,10 +176,8 @@ __device__ __forceinline__ bool absorbs(const Photon& p)
  * false when the roulette kills the photon.
  * MATH (TMC_MATH_*) selects how -ln u, sin/cos(2 pi u') and the analytic
  * sampler's sin(theta) are evaluated; everything else is the same
- * arithmetic in both modes.  STORED_KEYS takes the Philox round keys from
- * ph.keys; otherwise they are formed from the seed inside the step (fewer
- * live registers, for kernels short of scalar registers).  TABLE reads the
- * weight's images from wt (WeightTable).
+ * arithmetic in both modes.  The Philox round keys come from ph.keys
+ * (set_derived).  TABLE reads the weight's images from wt (WeightTable).
  * Rejection sampler (photon.c:34-43): attempt 1 uses words 1-2 of block
  * (k, 0), attempts 2 and 3 words 0-1 and 2-3 of block (k, 1), and so on.
  * An iteration makes at most the attempts of one block: a photon whose
@@ -197,7 +188,7 @@ __device__ __forceinline__ bool absorbs(const Photon& p)
  * computing a second block whenever one of its 64 lanes rejects (almost
  * every time: 1 - (pi/4)^64).  The arithmetic and its order per absorption
  * are unchanged. */
-template <uint32_t SAMPLER, uint32_t MATH, bool STORED_KEYS = true, bool TABLE = false>
+template <uint32_t SAMPLER, uint32_t MATH, bool TABLE = false>
 __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt,
                                      const WeightTable& wt = WeightTable{ nullptr, 0u, 0.0f })
 {
@@ -210,11 +201,10 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
         e = wt.entry[__builtin_elementwise_min(p.k, wt.k_end - 1u)];
     tmc_u32x4 r;
     if (SAMPLER == 1u) /* block (k, 0) for an absorption, (k, b) for attempts 2b and 2b + 1 */
-        r = tmc_philox4x32_10_blockb(p.k, (p.att & 0xFFu) ? ((p.att & 0xFFu) + 1u) >> 1 : 0u, p.pp,
-                                     STORED_KEYS ? &ph.keys : nullptr, ph.key0, ph.key1);
+        r = tmc_philox4x32_10_blockb(p.k, (p.att & 0xFFu) ? ((p.att & 0xFFu) + 1u) >> 1 : 0u, p.pp, &ph.keys,
+                                     ph.key0, ph.key1);
     else
-        r = STORED_KEYS ? tmc_philox4x32_10_block0_ks(p.k, p.pp, ph.keys) /* block (k, 0) */
-                        : tmc_philox4x32_10_block0(p.k, p.pp, ph.key0, ph.key1);
+        r = tmc_philox4x32_10_block0_ks(p.k, p.pp, ph.keys); /* block (k, 0) */
 
     bool cut = false; /* the weight after this absorption is below the cutoff */
     if (fresh) {

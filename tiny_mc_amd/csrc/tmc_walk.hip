@@ -114,7 +114,7 @@ __device__ __forceinline__ void flush_tallies(unsigned long long* lds, unsigned 
 template <uint32_t SAMPLER, uint32_t VARIANT, bool TABLE>
 __device__ __forceinline__ bool step_of(Photon& p, const Physics& ph, const LaneTally& lt, const WeightTable& wt)
 {
-    return step<SAMPLER, VARIANT, true, TABLE>(p, ph, lt, wt);
+    return step<SAMPLER, VARIANT, TABLE>(p, ph, lt, wt);
 }
 
 template <uint32_t SAMPLER, uint32_t VARIANT, bool TABLE>

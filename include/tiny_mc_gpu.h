@@ -95,7 +95,7 @@ typedef struct tmc_options {
     uint32_t weight_table; /* binary32 walk: the weight of a surviving photon is a function of
                               its absorption count, so the walk can read its fixed-point images
                               from a per-workgroup table instead of computing them (same bits).
-                              0 = automatic (analytic sampler, runs of 2^20 photons or more whose
+                              0 = automatic (runs of 2^20 photons or more, either sampler, whose
                               table spans the first roulette cycle and eight more, up to 1024
                               entries);
                               TMC_WEIGHT_TABLE_OFF = never; N = always, at most N entries */

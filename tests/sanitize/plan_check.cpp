@@ -82,15 +82,14 @@ void check_params_and_cadence()
         const double L = tmc::expected_interactions(&p, 0);
         CHECK(std::isfinite(L) && L >= 1.0);
         CHECK(tmc::expected_interactions(&p, 5) <= 5.0);
-        for (uint32_t prec = 0; prec < 2; ++prec)
-            for (uint32_t smp = 0; smp < 2; ++smp)
-                for (uint32_t req : { 0u, 1u, 700u, 5000u, TMC_WEIGHT_TABLE_OFF })
-                    for (uint64_t n : { (uint64_t)1, (uint64_t)1 << 20, (uint64_t)1 << 40 }) {
-                        const uint32_t e = tmc::weight_table_entries(&p, prec, smp, n, req);
-                        CHECK(e <= TMC_WALK_TABLE_MAX);
-                        if (prec != TMC_PRECISION_FP32 || req == TMC_WEIGHT_TABLE_OFF)
-                            CHECK(e == 0);
-                    }
+        for (uint32_t prec = 0; prec < 4; ++prec)
+            for (uint32_t req : { 0u, 1u, 700u, 5000u, TMC_WEIGHT_TABLE_OFF })
+                for (uint64_t n : { (uint64_t)1, (uint64_t)1 << 20, (uint64_t)1 << 40 }) {
+                    const uint32_t e = tmc::weight_table_entries(&p, prec, n, req);
+                    CHECK(e <= TMC_WALK_TABLE_MAX);
+                    if (prec != TMC_PRECISION_FP32 || req == TMC_WEIGHT_TABLE_OFF)
+                        CHECK(e == 0);
+                }
     }
     CHECK(tmc::photon_range_ok(0, 1ull << 63) && !tmc::photon_range_ok(1, 1ull << 63));
     CHECK(tmc::photon_range_ok(1ull << 63, 0) && !tmc::photon_range_ok((1ull << 63) + 1, 0));

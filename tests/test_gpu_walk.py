@@ -63,7 +63,7 @@ def test_weight_table_bit_exact(tmc, orc, hw_tables, math, sampler, phys, photon
     mirror (which multiplies) bit for bit, also for photons that outlive
     the table and continue with the arithmetic from its last weight."""
     tally, st = tmc.run_raw(_params(tmc, phys, photons), 606, sampler=sampler, math=math, weight_table=table)
-    assert st.weight_table == (table or (649 if sampler == 0 else 0))
+    assert st.weight_table == (table or 649)
     tables = hw_tables if math == tmc.MATH_HARDWARE else None
     want, n = _mirror(orc, phys, 606, photons, sampler=sampler, tables=tables)
     assert st.interactions == n
@@ -71,15 +71,15 @@ def test_weight_table_bit_exact(tmc, orc, hw_tables, math, sampler, phys, photon
 
 
 def test_weight_table_choice(tmc):
-    """Automatic table: on for analytic-sampler runs of 2^20 photons or more
-    whose table spans nine roulette cycles within 1024 entries, off
+    """Automatic table: on for runs of 2^20 photons or more (either
+    sampler) whose table spans nine roulette cycles within 1024 entries, off
     otherwise; never for the other precision modes."""
     big = 1 << 20
     assert tmc.run_raw(_params(tmc, DEFAULT, big), 1)[1].weight_table == 73 + 24 * 24
     assert tmc.run_raw(_params(tmc, DEFAULT, big - 64), 1)[1].weight_table == 0
     assert tmc.run_raw(_params(tmc, HIGHALB, big), 1, max_interactions=64)[1].weight_table == 0
     assert tmc.run_raw(_params(tmc, DEFAULT, big), 1, weight_table=tmc.WEIGHT_TABLE_OFF)[1].weight_table == 0
-    assert tmc.run_raw(_params(tmc, DEFAULT, big), 1, sampler=tmc.SAMPLER_REJECTION)[1].weight_table == 0
+    assert tmc.run_raw(_params(tmc, DEFAULT, big), 1, sampler=tmc.SAMPLER_REJECTION)[1].weight_table == 649
     assert tmc.run_raw(_params(tmc, DEFAULT, 4096), 1, precision=tmc.PRECISION_FP64, weight_table=64)[1].weight_table == 0
 
 

@@ -195,7 +195,7 @@ int tmc::run_walk(Context* c, const tmc_params* p, uint64_t seed, const tmc_opti
         kcap = 0xFFFFFFFFull;
     if (o->reserved)
         return TMC_E_INVALID;
-    const uint32_t table_k = tmc::weight_table_entries(p, o->precision, o->sampler, p->photons, o->weight_table);
+    const uint32_t table_k = tmc::weight_table_entries(p, o->precision, p->photons, o->weight_table);
     uint32_t grid = o->grid;
     if (grid == 0) {
         int per_cu = 0;

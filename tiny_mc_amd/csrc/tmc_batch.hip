@@ -132,7 +132,10 @@ __device__ __forceinline__ unsigned long long take_sys(unsigned long long* p)
  * through an empty asm, so each rare path loads the fields it needs (scalar
  * loads, cached) and no load is hoisted out of it: the serving kernel's loop
  * then holds no argument in scalar registers, which leaves room for the
- * set's Philox round keys.  Batch: the arguments as passed. */
+ * set's Philox round keys.  Batch: the arguments as passed.  The argument
+ * block starts with the BatchArgs the kernel takes by value (explicit
+ * kernel arguments are laid out from offset 0 in declaration order), so
+ * walk_multi must keep it as its first parameter. */
 template <bool SERVE>
 __device__ __forceinline__ const BatchArgs& args_at_use(const BatchArgs& a)
 {

@@ -48,8 +48,7 @@ double expected_interactions(const tmc_params* p, uint64_t kcap);
 
 /* Entries of the single run's weight table for these parameters and the
  * caller's request (tmc_options.weight_table; 0: none). */
-uint32_t weight_table_entries(const tmc_params* p, uint32_t precision, uint32_t sampler, uint64_t photons,
-                              uint32_t request);
+uint32_t weight_table_entries(const tmc_params* p, uint32_t precision, uint64_t photons, uint32_t request);
 
 /* --- batched sweep ------------------------------------------------------ */
 

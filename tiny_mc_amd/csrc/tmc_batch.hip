@@ -27,8 +27,9 @@
  *   - the request table, the chunk ring and the control words (tail, stop)
  *     live in host memory; waves read them with system-scope loads, the
  *     tail through a device-memory cache that one wave at a time refreshes;
- *   - a wave takes a ticket (ring index) and keeps it until the tail passes
- *     it; a wave leaves only when the host has asked this launch to stop,
+ *   - a wave whose lanes have drained takes a ticket (ring index) and keeps
+ *     it until the tail passes it; a wave leaves only when the host has
+ *     asked this launch to stop,
  *     which the host does only when every published chunk is done;
  *   - the wave whose completion count reaches a request's photons moves
  *     the request's sums (and absorptions) into a host mirror, zeroes them
@@ -196,9 +197,9 @@ enum ClaimResult { CLAIM_GOT, CLAIM_EMPTY, CLAIM_DONE };
 
 /* Tries to give the wave its next chunk (wave-uniform result).
  * Batch: the next index of the fixed queue, or DONE past its end.
- * Serving: the wave takes a ticket (the next ring index) once and keeps it
- * until its entry is published; meanwhile the call returns EMPTY without
- * waiting, so the wave's live lanes keep walking.  The tail is read from
+ * Serving: the wave (drained) takes a ticket (the next ring index) once and
+ * keeps it until its entry is published; meanwhile the call returns EMPTY
+ * without waiting, and the wave backs off.  The tail is read from
  * the wave's copy of the control words (`group`); when the ticket lies past
  * it and the last refresh is older than the refresh interval, the one wave
  * of group 0 that wins the refresh reads the host's words (stop first, with
@@ -415,9 +416,8 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
      * whose photon ended cost scalar instructions on every step) */
     uint32_t absorbed = 0, steps = 0, launched = 0, live_pub = 0;
     uint64_t ticket = ~0ull; /* serving: ring index held, ~0 if none */
-    /* serving: steps before re-polling an unpublished ticket (low 16 bits)
-     * and the number of failed polls in a row (high bits), one scalar word */
-    uint32_t poll = 0;
+    /* serving: failed polls in a row of a dry wave (its back-off) */
+    uint32_t fails = 0;
     const uint32_t group = SERVE ? (blockIdx.x * (blockDim.x >> 6) + wave) % TMC_SERVE_CACHE_COPIES : 0u;
 
     /* a lane holds a live photon while its weight is >= 0; -1 marks a lane
@@ -436,13 +436,20 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             /* the pool is empty: claim a chunk, or switch sets once drained
              * (PENDING is set by a claim with the pool empty and stays so
              * until the switch) */
-            if (SERVE && (poll & 0xFFFFu) && dead != ~0ull) {
-                --poll; /* live lanes keep walking; poll again later */
+            if (SERVE && dead != ~0ull) {
+                /* serving: a wave takes its next ticket only once its lanes
+                 * have drained.  Reserved earlier, a chunk of another request
+                 * (the usual case: most requests are one chunk or a few)
+                 * waits out the drain, milliseconds for long walks, while
+                 * idle waves could walk it: measured at half load, p50
+                 * 10.3 -> 6.0 ms and p99 64 -> 56 ms, for 1% of the capacity
+                 * at saturation (a wave whose next chunk would continue its
+                 * request now drains first) */
             } else if (!(flags & (PENDING | QUEUE_DONE))) {
                 BatchChunk ch;
                 const ClaimResult r = try_claim<SERVE>(a, lane, group, ticket, ch);
                 if (r == CLAIM_GOT) {
-                    poll = 0;
+                    fails = 0;
                     if (ch.set == cur_set) {
                         if (SERVE && ch.count == 0) {
                             publish_skip(a, st, lane, ch.first); /* cancelled photons */
@@ -456,8 +463,6 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                     }
                 } else if (r == CLAIM_DONE) {
                     flags |= QUEUE_DONE;
-                } else if (SERVE && dead != ~0ull) {
-                    poll = (poll & 0xFFFF0000u) | 16u;
                 } else if (SERVE) {
                     /* ran dry: publish what was completed, then wait a
                      * little.  A wave never leaves on its own: the ticket it
@@ -473,10 +478,9 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                     /* back off while dry: 1.7 us after the first failed
                      * poll, doubling to 27 us (a new chunk waits at most
                      * that long for an idle wave) */
-                    const uint32_t fails = poll >> 16;
-                    for (uint32_t k = 0; k < (1u << (fails < 4u ? fails : 4u)); ++k)
+                    for (uint32_t k = 0; k < (1u << fails); ++k)
                         __builtin_amdgcn_s_sleep(64);
-                    poll = (fails < 4u ? fails + 1u : 4u) << 16;
+                    fails = fails < 4u ? fails + 1u : 4u;
                 }
             }
             if ((flags & PENDING) && dead == ~0ull) {

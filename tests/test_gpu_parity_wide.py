@@ -1,4 +1,4 @@
-"""Parity beyond default physics, at the north-star tolerance bar, and
+"""Parity beyond default physics, at the max(3 sigma, 1e-5 relative) bar, and
 size-independent properties of the full-size run (GPU tests).
 
 - FP32 GPU walk vs FP64 oracle (orc_walk64, same Philox stream): every

@@ -309,7 +309,7 @@ def test_tiny_ring_wraps_many_times(tmc):
 
 
 def test_served_results_within_tolerance_of_fp64_oracle(tmc, orc):
-    """Per-request accuracy at the north star's bar: each served result's
+    """Per-request accuracy at the single run's tolerance bar: each served result's
     heat, shell by shell, within max(3 sigma, 1e-5 relative) of the FP64
     oracle on the same stream, for requests of four physics served together
     (sigma: the larger of the heat2 estimate and the standard error over 8

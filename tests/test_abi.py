@@ -96,7 +96,7 @@ def test_tally_to_heat_matches_independent_conversion(tmc):
 
 
 def test_driver_binaries_link_the_library(tmc):
-    for name in ("tiny_mc", "tiny_mc_seed12345"):
+    for name in ("tiny_mc", "tiny_mc_seed12345", "tiny_mc_seed12345_rej"):
         exe = os.path.join(ROOT, "tiny_mc_amd", name)
         assert os.path.exists(exe), exe
         out = subprocess.run(["ldd", exe], capture_output=True, text=True, check=True).stdout

@@ -14,8 +14,11 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_driver_table_matches_mirror(tmc, orc):
-    exe = os.path.join(ROOT, "tiny_mc_amd", "tiny_mc_seed12345")  # SEED=12345, PHOTONS=65536
+@pytest.mark.parametrize("exe_name,sampler", [("tiny_mc_seed12345", 0), ("tiny_mc_seed12345_rej", 1)])
+def test_driver_table_matches_mirror(tmc, orc, exe_name, sampler):
+    """SEED=12345, PHOTONS=65536; the _rej build (-DSAMPLER=1) walks with the
+    reference's rejection sampler."""
+    exe = os.path.join(ROOT, "tiny_mc_amd", exe_name)
     out = subprocess.run([exe], capture_output=True, text=True, check=True, timeout=120).stdout
     lines = out.splitlines()
     assert lines[0] == "# Tiny Monte Carlo by Scott Prahl (http://omlc.ogi.edu)"
@@ -31,7 +34,8 @@ def test_driver_table_matches_mirror(tmc, orc):
     assert len(rows) == 100 and lines[-1].startswith("# extra\t")
     got = np.array([[float(v) for v in r] for r in rows])
 
-    tally, _ = orc.mirror32(orc.params(), 12345, 65536, tables=tmc.step_math_tables(tmc.MATH_HARDWARE))
+    tally, _ = orc.mirror32(orc.params(), 12345, 65536, sampler=sampler,
+                            tables=tmc.step_math_tables(tmc.MATH_HARDWARE))
     heat, heat2 = oracle_lib.tally_heat(tally, 101, 2.0, 20.0)
     rows_want, extra = tmc.table_rows(heat, heat2, 65536, 50.0)
     want = np.array(rows_want)

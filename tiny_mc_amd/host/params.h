@@ -31,3 +31,10 @@
 #ifndef SEED
 #define SEED (time(NULL)) /* stream seed */
 #endif
+
+/* Not in the reference: the direction sampler of the GPU walk.  0 (default)
+ * draws cos(theta) and phi directly; 1 is the reference's rejection method
+ * (photon.c:34-43).  Both give the same isotropic distribution. */
+#ifndef SAMPLER
+#define SAMPLER 0
+#endif

@@ -7,8 +7,9 @@
  * (tiny_mc.c:43-49) becomes one tiny_mc_gpu_run() call, and the tallies
  * arrive in double, so the normalisation is done in double throughout.
  *
- * Build: make -C tiny_mc_amd driver (optionally with -D overrides in
- * DRIVER_DEFS, e.g. DRIVER_DEFS='-DSEED=12345 -DPHOTONS=1073741824').
+ * Build: make -C tiny_mc_amd tiny_mc (optionally with -D overrides in
+ * DRIVER_DEFS, e.g. DRIVER_DEFS='-DSEED=12345 -DPHOTONS=1073741824';
+ * -DSAMPLER=1 selects the reference's rejection sampler).
  */
 #define _XOPEN_SOURCE 500 /* M_PI */
 
@@ -28,6 +29,9 @@ static const char t3[] = "MI355X version: HIP gfx950 walk kernel behind the tiny
 
 static double heat[SHELLS];
 static double heat2[SHELLS];
+#if SAMPLER != 0
+static uint64_t tally[4 * SHELLS];
+#endif
 
 int main(void)
 {
@@ -42,7 +46,18 @@ int main(void)
     const unsigned long long seed = (unsigned long long)(SEED);
 
     double start = wtime();
+#if SAMPLER != 0
+    /* the reference's rejection sampler: the call with options, then the
+     * integer tallies in the reference's units */
+    tmc_options opt;
+    tmc_options_default(&opt);
+    opt.sampler = (uint32_t)(SAMPLER);
+    int rc = tiny_mc_gpu_run_ex(&params, seed, &opt, tally, NULL);
+    if (rc == TMC_OK)
+        rc = tiny_mc_gpu_tally_to_heat(&params, tally, heat, heat2);
+#else
     int rc = tiny_mc_gpu_run(&params, seed, heat, heat2);
+#endif
     double end = wtime();
     if (rc != TMC_OK) {
         fprintf(stderr, "tiny_mc: %s\n", tiny_mc_gpu_strerror(rc));

@@ -42,15 +42,22 @@
 
 namespace tmc {
 
-/* Copies of the overflow shell's sums in each wave's slot (lane & 15 picks
- * one), after the slot's shell copies.  Shells and copies alike are
- * (w image, w*w image) pairs of adjacent words. */
+/* Copies of the overflow shell's sums in each copy of a wave's slot
+ * (lane & 15 picks one), after the inner shells (LaneTally).  Shells and
+ * copies alike are (w image, w*w image) pairs of adjacent words. */
 #define TMC_SLOT_HOT 16u
+
+/* 64-bit words of one copy of a slot: room for the inner shells of a set of
+ * up to shells_max shells and its overflow copies. */
+__host__ __device__ __forceinline__ size_t slot_copy_words(uint32_t shells_max)
+{
+    return 2u * ((size_t)shells_max - 1u + TMC_SLOT_HOT);
+}
 
 /* 64-bit words of one wave's LDS slot. */
 __host__ __device__ __forceinline__ size_t slot_words(uint32_t shells_max, uint32_t copies)
 {
-    return (size_t)copies * 2u * shells_max + 2u * TMC_SLOT_HOT;
+    return (size_t)copies * slot_copy_words(shells_max);
 }
 
 /* Moves one wave's private LDS slot into a set's global sums and zeroes it.
@@ -62,26 +69,17 @@ template <bool SERVE>
 __device__ __forceinline__ void flush_slot(unsigned long long* slot, uint32_t shells_max, uint32_t copies,
                                            uint32_t shells, unsigned long long* g, uint32_t lane)
 {
-    unsigned long long* hot = slot + (size_t)copies * 2 * shells_max;
-    for (uint32_t s = lane; s < shells + TMC_SLOT_HOT; s += 64) {
+    const uint32_t last = shells - 1;
+    for (uint32_t q = lane; q < last + TMC_SLOT_HOT; q += 64) {
         unsigned long long a = 0, b = 0;
-        uint32_t shell = s;
-        if (s < shells) {
-            for (uint32_t c = 0; c < copies; ++c) {
-                unsigned long long* lw = slot + (size_t)c * 2 * shells_max;
-                a += lw[2 * s];
-                b += lw[2 * s + 1];
-                lw[2 * s] = 0;
-                lw[2 * s + 1] = 0;
-            }
-        } else {
-            const uint32_t h = s - shells;
-            a = hot[2 * h];
-            b = hot[2 * h + 1];
-            hot[2 * h] = 0;
-            hot[2 * h + 1] = 0;
-            shell = shells - 1;
+        for (uint32_t c = 0; c < copies; ++c) {
+            unsigned long long* lw = slot + (size_t)c * slot_copy_words(shells_max);
+            a += lw[2 * q];
+            b += lw[2 * q + 1];
+            lw[2 * q] = 0;
+            lw[2 * q + 1] = 0;
         }
+        const uint32_t shell = q < last ? q : last;
         if (SERVE) {
             if (a)
                 add128_sys(g + 2 * shell, a);
@@ -397,9 +395,8 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
     unsigned long long* slot = lds_slots + (size_t)wave * slot_words(a.shells_max, a.copies);
     for (uint32_t i = lane; i < slot_words(a.shells_max, a.copies); i += 64)
         slot[i] = 0;
-    unsigned long long* lw = slot + (a.copies == 2 && lane >= 32 ? 2 * a.shells_max : 0);
-    unsigned long long* hot = slot + (size_t)a.copies * 2 * a.shells_max + 2 * (lane & (TMC_SLOT_HOT - 1));
-    const LaneTally lt = { lw, hot };
+    /* fmax is set with each set (its shell_max) */
+    LaneTally lt = { slot + (a.copies == 2 && lane >= 32 ? slot_copy_words(a.shells_max) : 0), 0.0f, ~0ull };
     const uint32_t slot_mask = SERVE ? (1u << TMC_SERVE_SLOT_BITS) - 1u : 0xFFFFFFFFu;
 
     SetState st;
@@ -492,6 +489,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 }
                 cur_set = pend.set;
                 load_set<SAMPLER, SERVE>(st, a, cur_set & slot_mask, lane);
+                lt.fmax = tally_fmax(st.phys.shell_max, lane & (TMC_SLOT_HOT - 1));
                 /* every lane is empty here: only a small cap needs checks */
                 flags = st.kcap <= TMC_SLOT_FLUSH_STEPS ? (flags | CAPPING) : (flags & ~CAPPING);
                 if (SERVE && pend.count == 0) {
@@ -554,7 +552,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 flush_and_publish<SERVE>(slot, a, st, lane, completed, absorbed, steps);
                 launched = 0;
                 live_pub = live_now;
-                flags = cap_within(ph, st.kcap, TMC_SLOT_FLUSH_STEPS) ? (flags | CAPPING) : (flags & ~CAPPING);
+                flags = cap_within(ph, ph.weight >= 0, st.kcap, TMC_SLOT_FLUSH_STEPS) ? (flags | CAPPING) : (flags & ~CAPPING);
             }
         }
     }

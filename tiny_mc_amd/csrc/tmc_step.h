@@ -30,13 +30,12 @@ struct Physics {
     float spm; /* shells per mean free path (photon.c:9) */
     float shell_max; /* (float)(shells - 1), the overflow shell */
     uint32_t key0, key1; /* Philox key = 64-bit seed */
-    float far_min; /* shell_max + 1, set by set_derived */
     float clear_half; /* 0.5 - eps, set by set_derived */
     tmc_philox_keys keys; /* round keys of (key0, key1), set by set_derived */
 };
 
 /* Per-set constants derived on the device: the Philox round keys, and the
- * bounds of shell_index for the set's shell_max: a shell coordinate computed
+ * bound of shell_index for the set's shell_max: a shell coordinate computed
  * from the hardware square root is within 3 ulp of the exact one (below), so
  * it has the exact floor if it lies more than eps = 8 ulp from every integer,
  * with the ulp of the floats just below 2^e > shell_max + 2; at or beyond
@@ -58,39 +57,82 @@ __device__ __forceinline__ void set_derived(Physics& ph)
         }
     }
     const int e = __builtin_amdgcn_frexp_expf(ph.shell_max + 2.0f); /* shell_max + 2 < 2^e */
-    /* wave-uniform: keep the bounds in scalar registers */
-    ph.far_min = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(ph.shell_max + 1.0f)));
+    /* wave-uniform: keep the bound in a scalar register */
     ph.clear_half =
         __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(0.5f - __builtin_ldexpf(1.0f, e - 21))));
 }
 
-/* The short shell coordinate f' = RN(v_sqrt(r2) * spm) and whether its floor
- * and its side of shell_max are certainly those of the exact coordinate. */
-__device__ __forceinline__ float shell_coord_short(float r2, const Physics& ph, bool* sure)
+/* LDS words one lane deposits into.  Each pair of adjacent words holds a
+ * shell's two sums (w image, w*w image), so one address serves both atomics
+ * (the second takes the instruction's offset field).  Pairs 0 .. shell_max - 1
+ * are the inner shells; the overflow shell (shell_max, the last one) has H
+ * copies, pairs shell_max .. shell_max + H - 1: a long walk ends up there
+ * most of the time, where the lanes of a wave would all hit one pair.  A lane
+ * deposits at pair floor(min(f, fmax)), f its shell coordinate and
+ * fmax = shell_max + 1/2 + (its copy, lane mod H): the clamp that bounds the
+ * index also spreads the overflow deposits over the copies, which are added
+ * up when the sums are flushed. */
+struct LaneTally {
+    unsigned long long* base; /* pair q at base[2 q], base[2 q + 1] */
+    float fmax; /* shell_max + 0.5 + this lane's overflow copy */
+    /* wave-uniform: the lanes whose deposits count (a walk may step lanes
+     * that hold no photon, so that it never branches around the step) */
+    uint64_t deposit;
+};
+
+/* fmax of a lane for copy `copy` of the overflow shell. */
+__device__ __forceinline__ float tally_fmax(float shell_max, uint32_t copy) { return shell_max + 0.5f + (float)copy; }
+
+/* Adds one deposit's images to pair q (lanes in lt.deposit only). */
+__device__ __forceinline__ void tally_add(const LaneTally& lt, uint32_t q, unsigned long long a, unsigned long long b)
 {
-    const float f = __builtin_amdgcn_sqrtf(r2) * ph.spm;
-    *sure = f >= ph.far_min || __builtin_fabsf(__builtin_amdgcn_fractf(f) - 0.5f) < ph.clear_half;
-    return f;
+    if (__builtin_amdgcn_inverse_ballot_w64(lt.deposit)) {
+        unsigned long long* p = lt.base + 2u * q;
+        atomicAdd(p, a);
+        atomicAdd(p + 1, b);
+    }
 }
 
-/* Shell of a deposit at squared radius r2 (photon.c:26-29): with
- * f = RN(RN(sqrt(r2)) * spm), whether f < shell_max (else the overflow
- * shell) and floor(f).  The hardware square root is within one ulp of the
- * correctly rounded one (the one-ulp correction of sqrt_cr_walk is exact on
- * every float), so f' = RN(v_sqrt(r2) * spm) is within 3 ulp of f and has
- * the same floor and the same side of shell_max when it is clear of every
- * integer by eps or lies at or beyond shell_max + 1 (checked on every float
- * r2 by tiny_mc_gpu_check_shell_index).  Otherwise, for a lane near a shell
- * boundary (a few per ten thousand at 101 shells, including radii within
- * eps / spm of the origin), the whole wave takes the IEEE square root. */
-__device__ __forceinline__ uint32_t shell_index(float r2, const Physics& ph, bool* inner)
+/* The pair of shell s (inner) or of this lane's overflow copy, for a shell
+ * coordinate computed exactly. */
+__device__ __forceinline__ uint32_t tally_pair(const LaneTally& lt, bool inner, uint32_t s)
+{
+    return inner ? s : (uint32_t)lt.fmax;
+}
+
+/* The clamped short shell coordinate fc = min(RN(v_sqrt(r2) * spm), fmax)
+ * and whether its floor is certainly the pair of the exact coordinate. */
+__device__ __forceinline__ float shell_coord_short(float r2, const Physics& ph, float fmax, bool* sure)
+{
+    /* both operands are non-negative floats, ordered as their bit patterns:
+     * one integer min, where fminf would first quiet a signalling fmax */
+    const float fc = __uint_as_float(
+        __builtin_elementwise_min(__float_as_uint(__builtin_amdgcn_sqrtf(r2) * ph.spm), __float_as_uint(fmax)));
+    *sure = __builtin_fabsf(__builtin_amdgcn_fractf(fc) - 0.5f) < ph.clear_half;
+    return fc;
+}
+
+/* Tally pair of a deposit at squared radius r2 (photon.c:26-29): with
+ * f = RN(RN(sqrt(r2)) * spm), floor(f) if f < shell_max, else one of the
+ * overflow shell's copies.  The hardware square root is within one ulp of
+ * the correctly rounded one (the one-ulp correction of sqrt_cr_walk is exact
+ * on every float), so f' = RN(v_sqrt(r2) * spm) is within 3 ulp of f.  The
+ * clamped fc = min(f', fmax) has the right pair when it is clear of every
+ * integer by eps: below shell_max + 1, floor(f') = floor(f) and both lie on
+ * the same side of shell_max; from shell_max + 1 on (fmax included, whose
+ * fraction is 1/2), f' and f are both overflows and any copy will do
+ * (checked on every float r2 and every copy by tiny_mc_gpu_check_shell_index).
+ * Otherwise, for a lane near a shell boundary (a few per ten thousand at 101
+ * shells, including radii within eps / spm of the origin), the whole wave
+ * takes the IEEE square root. */
+__device__ __forceinline__ uint32_t shell_index(float r2, const Physics& ph, float fmax)
 {
     bool sure;
-    float f = shell_coord_short(r2, ph, &sure);
+    float fc = shell_coord_short(r2, ph, fmax, &sure);
     if (__builtin_expect(__ballot(!sure) != 0, 0))
-        f = sure ? f : sqrt_rn(r2) * ph.spm;
-    *inner = f < ph.shell_max;
-    return (uint32_t)f;
+        fc = sure ? fc : __uint_as_float(__builtin_elementwise_min(__float_as_uint(sqrt_rn(r2) * ph.spm),
+                                                                    __float_as_uint(fmax)));
+    return (uint32_t)fc;
 }
 
 __device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon, uint32_t key0, uint32_t key1)
@@ -107,54 +149,57 @@ __device__ __forceinline__ void launch_photon(Photon& p, uint64_t photon, uint32
     p.pp = tmc_philox_photon_init(p.plo, p.phi, key0, key1);
 }
 
-/* LDS words one lane deposits into.  Each shell's two sums (w image, w*w
- * image) are adjacent words, 2 s and 2 s + 1, so one address serves both
- * atomics (the second takes the instruction's offset field).  A long walk
- * ends up in the overflow shell (the last one) most of the time, where the
- * lanes of a wave would all hit one pair; each lane group has its own copy
- * of that pair instead, laid out the same way, and the copies are added up
- * when the sums are flushed. */
-struct LaneTally {
-    unsigned long long* shell; /* pair of shell s at shell[2 s], shell[2 s + 1] */
-    unsigned long long* hot; /* this lane's copy of the overflow shell's pair */
-};
-
-/* Adds one deposit's images to shell s (inner) or to the overflow pair. */
-__device__ __forceinline__ void tally_add(const LaneTally& lt, bool inner, uint32_t s, unsigned long long a,
-                                          unsigned long long b)
-{
-    unsigned long long* p = inner ? lt.shell + 2u * s : lt.hot;
-    atomicAdd(p, a);
-    atomicAdd(p + 1, b);
-}
-
 /* The binary32 weight sequence of one parameter set, as a table in LDS.
  * A photon's weight is a function of its absorption count k alone: every
  * photon starts at 1, multiplies by the albedo at each absorption
  * (photon.c:32) and, once below the cutoff, either dies or continues at the
  * weight divided by 0.1 (photon.c:45-49), the same for every survivor.
  * Entry k holds the fixed-point images of the weight W[k] deposited at
- * absorption k (fx_heat, fx_heat2) and, in bit 31 of word 1, whether
- * RN(W[k] a) falls below the cutoff.  Photons at k >= k_end (rare: the table
- * spans many roulette cycles) fall back to the arithmetic, starting from
- * w_end = W[k_end]. */
+ * absorption k (fx_heat, fx_heat2: words 0-1 and 2-3, low word first).
+ * Whether RN(w a) falls below the cutoff, for any weight w in [0, 1], is
+ * fx_heat(w) < h_cut = fx_heat(T), T the least float with RN(T a) >= 0.001f:
+ * RN(w a) is monotone in w, so the weights that fall below are those under
+ * T, and T > 2^-10 (a < 1); fx_heat is exact, hence strictly increasing, on
+ * [2^-10, 1], and below 2^23 = its image of 2^-10 underneath.  Photons at
+ * k >= k_end (rare: the table spans many roulette cycles) fall back to the
+ * arithmetic, starting from w_end = W[k_end]. */
 struct WeightTable {
     const uint4* entry;
     uint32_t k_end;
     float w_end;
+    uint64_t h_cut;
+    /* wave-uniform: some photon of the wave may reach k_end before the walk
+     * evaluates this again (table_near_end); while false, the step skips
+     * the test of its photons against k_end */
+    bool near_end;
 };
 
-/* Whether a live photon of the wave can reach the absorption cap kcap
- * within the next `steps` steps: its k grows by one per step, and photons
+/* Steps between evaluations of WeightTable::near_end. */
+#define TMC_TABLE_CHECK_STEPS 64u
+
+/* Whether a live photon of the wave (live: this lane holds one) can reach
+ * k_end within the next TMC_TABLE_CHECK_STEPS steps: its k grows by at most
+ * one per step, and photons launched meanwhile start at k = 0. */
+template <class PhotonT>
+__device__ __forceinline__ bool table_near_end(const PhotonT& ph, bool live, uint32_t k_end)
+{
+    if (k_end <= TMC_TABLE_CHECK_STEPS)
+        return true;
+    return __ballot(live && ph.k >= k_end - TMC_TABLE_CHECK_STEPS) != 0;
+}
+
+/* Whether a live photon of the wave (live: this lane holds one) can reach
+ * the absorption cap kcap within the next `steps` steps: its k grows by one
+ * per step, and photons
  * launched meanwhile start at k = 0.  Evaluated at each periodic flush, it
  * lets the walk skip the per-step cap check until the next one; with the
  * default cap (2^32 - 1) the check never runs. */
 template <class PhotonT>
-__device__ __forceinline__ bool cap_within(const PhotonT& ph, uint32_t kcap, uint32_t steps)
+__device__ __forceinline__ bool cap_within(const PhotonT& ph, bool live, uint32_t kcap, uint32_t steps)
 {
     if (kcap <= steps)
         return true;
-    return __ballot(ph.weight >= 0 && ph.k >= kcap - steps) != 0;
+    return __ballot(live && ph.k >= kcap - steps) != 0;
 }
 
 /* The roulette kills a photon when u > 0.1f (photon.c:46).  For
@@ -188,9 +233,16 @@ __device__ __forceinline__ bool absorbs(const Photon& p)
  * computing a second block whenever one of its 64 lanes rejects (almost
  * every time: 1 - (pi/4)^64).  The arithmetic and its order per absorption
  * are unchanged. */
-template <uint32_t SAMPLER, uint32_t MATH, bool TABLE = false>
-__device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt,
-                                     const WeightTable& wt = WeightTable{ nullptr, 0u, 0.0f })
+/* What the roulette of a step reads: whether the weight after this
+ * absorption fell below the cutoff (false while a rejection attempt is
+ * pending) and the roulette word. */
+struct StepEnd {
+    bool cut;
+    uint32_t x3;
+};
+
+template <uint32_t SAMPLER, uint32_t MATH, bool TABLE>
+__device__ __forceinline__ StepEnd step_core(Photon& p, const Physics& ph, const LaneTally& lt, const WeightTable& wt)
 {
     const bool fresh = absorbs<SAMPLER>(p);
     /* the table entry first: its load overlaps the generator (where exactly
@@ -216,12 +268,10 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
 
         /* absorb (photon.c:26-32) */
         const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
-        bool inner; /* else the overflow shell */
-        const uint32_t shell = shell_index(r2, ph, &inner);
+        const uint32_t q = shell_index(r2, ph, lt.fmax);
         if (TABLE) {
-            uint64_t h = ((uint64_t)(e.y & 0x7FFFFFFFu) << 32) | e.x, h2 = ((uint64_t)e.w << 32) | e.z;
-            cut = (int32_t)e.y < 0;
-            if (__builtin_expect(__ballot(p.k >= wt.k_end) != 0, 0)) {
+            uint64_t h = ((uint64_t)e.y << 32) | e.x, h2 = ((uint64_t)e.w << 32) | e.z;
+            if (__builtin_expect(wt.near_end, 0) && __builtin_expect(__ballot(p.k >= wt.k_end) != 0, 0)) {
                 /* photons past the table continue with the arithmetic */
                 if (p.k >= wt.k_end) {
                     if (p.k == wt.k_end)
@@ -229,12 +279,16 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
                     h = fx_heat(p.weight);
                     h2 = fx_heat2(p.weight);
                     p.weight = p.weight * ph.albedo;
-                    cut = p.weight < 0.001f;
+                    /* the weight a roulette survivor continues with (a
+                     * photon the roulette kills drops it) */
+                    if (p.weight < 0.001f)
+                        p.weight = div_tenth(p.weight); /* = p.weight / 0.1f */
                 }
             }
-            tally_add(lt, inner, shell, h, h2);
+            cut = h < wt.h_cut; /* = RN(w a) < 0.001f for the deposited weight w */
+            tally_add(lt, q, h, h2);
         } else {
-            tally_add(lt, inner, shell, fx_heat(p.weight), fx_heat2(p.weight));
+            tally_add(lt, q, fx_heat(p.weight), fx_heat2(p.weight));
             p.weight = p.weight * ph.albedo;
             cut = p.weight < 0.001f;
         }
@@ -263,7 +317,7 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
             } else {
                 p.att = (p.att & 0x80000000u) | att;
             }
-            return true;
+            return StepEnd{ false, 0u };
         }
         if (!fresh) {
             cut = (p.att & 0x80000000u) != 0u;
@@ -284,15 +338,37 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
         p.w = s * sn;
     }
     ++p.k;
+    return StepEnd{ cut, x3 };
+}
 
-    /* roulette (photon.c:45-49) */
-    if (cut) {
-        if (x3 >= TMC_ROULETTE_KILL_R) /* u > 0.1f */
+/* One iteration of p (step_core), then the roulette (photon.c:45-49).
+ * Returns false when the roulette kills the photon. */
+template <uint32_t SAMPLER, uint32_t MATH, bool TABLE = false>
+__device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt,
+                                     const WeightTable& wt = WeightTable{ nullptr, 0u, 0.0f, 0ull, false })
+{
+    const StepEnd e = step_core<SAMPLER, MATH, TABLE>(p, ph, lt, wt);
+    if (e.cut) {
+        if (e.x3 >= TMC_ROULETTE_KILL_R) /* u > 0.1f */
             return false;
-        if (!TABLE || p.k > wt.k_end) /* the weight is the arithmetic's (p.k counts this absorption) */
+        if (!TABLE) /* (with the table, a photon past it has divided already) */
             p.weight = div_tenth(p.weight); /* = p.weight / 0.1f */
     }
     return true;
+}
+
+/* step() for a whole wave whose lanes all step (those without a photon
+ * with their deposits masked): returns the lanes whose photon the roulette
+ * kills, as the ballots of the two compares it reads.  (The ballot of
+ * step()'s result costs two vector instructions: the compiler rebuilds a
+ * compare from the merged lane mask.) */
+template <uint32_t SAMPLER, uint32_t MATH, bool TABLE>
+__device__ __forceinline__ uint64_t step_wave(Photon& p, const Physics& ph, const LaneTally& lt, const WeightTable& wt)
+{
+    const StepEnd e = step_core<SAMPLER, MATH, TABLE>(p, ph, lt, wt);
+    if (!TABLE && e.cut) /* (a photon the roulette kills drops its weight) */
+        p.weight = div_tenth(p.weight); /* = p.weight / 0.1f */
+    return __ballot(e.cut) & __ballot(e.x3 >= TMC_ROULETTE_KILL_R); /* u > 0.1f */
 }
 
 } // namespace tmc

@@ -128,8 +128,7 @@ __device__ __forceinline__ bool step(PhotonW& p, const PhysicsW& ph, const LaneT
     /* absorb (photon.c:26-32) */
     const float r2 = __builtin_fmaf(p.z, p.z, __builtin_fmaf(p.y, p.y, p.x * p.x));
     const float f = sqrt_radius(r2) * ph.spm;
-    const bool inner = f < ph.shell_max;
-    tally_add(lt, inner, (uint32_t)f, fx_heat(p.weight), fx_heat2(p.weight));
+    tally_add(lt, tally_pair(lt, f < ph.shell_max, (uint32_t)f), fx_heat(p.weight), fx_heat2(p.weight));
     ++p.e;
     p.weight = weight_from_counts(p.e, p.r, ph);
 
@@ -227,8 +226,7 @@ __device__ __forceinline__ bool step(Photon64& p, const Physics64& ph, const Lan
 
     /* absorb (photon.c:26-32) */
     const double f = sqrt(p.x * p.x + p.y * p.y + p.z * p.z) * ph.spm;
-    const bool inner = f < ph.shell_max;
-    tally_add(lt, inner, (uint32_t)f, fx_heat64(p.weight), fx_heat2_64(p.weight));
+    tally_add(lt, tally_pair(lt, f < ph.shell_max, (uint32_t)f), fx_heat64(p.weight), fx_heat2_64(p.weight));
     p.weight *= ph.albedo;
 
     /* new direction (photon.c:34-43) */

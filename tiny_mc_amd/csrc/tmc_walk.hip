@@ -34,31 +34,45 @@
 namespace tmc {
 
 /* Copies of the overflow shell's sums per workgroup (lane & 31 picks one):
- * 32 consecutive 64-bit words cover the LDS banks once. */
+ * 32 consecutive pairs of words. */
 #define TMC_WALK_HOT 32u
 
-/* LDS words of one workgroup: w and w*w sums per shell, then the
- * overflow shell's copies of both (a multiple of 2 words: the table that
- * follows is 16-byte aligned). */
-__host__ __device__ __forceinline__ size_t walk_lds_words(uint32_t shells) { return 2u * shells + 2u * TMC_WALK_HOT; }
+/* Dead lanes a wave gathers before it launches photons into them: a launch
+ * costs the wave about fifteen vector instructions whatever the number of
+ * lanes it fills, and the lanes wait a few steps in exchange.  Measured at
+ * 2^30 photons, default physics: 1 -> 149.9 ms, 2 -> 144.4, 4 -> 143.6,
+ * 8 -> 143.6. */
+#ifndef TMC_WALK_RELAUNCH_MIN
+#define TMC_WALK_RELAUNCH_MIN 4
+#endif
+
+/* LDS words of one workgroup: the (w, w*w) sum pairs of the inner shells,
+ * then the overflow shell's copies (LaneTally; a multiple of 2 words: the
+ * table that follows is 16-byte aligned). */
+__host__ __device__ __forceinline__ size_t walk_lds_words(uint32_t shells)
+{
+    return 2u * ((size_t)shells - 1u + TMC_WALK_HOT);
+}
 
 /* LDS bytes of a workgroup: the tallies, then table_k table entries (16
  * bytes) and the weight sequence they are built from (table_k + 1 floats). */
 __host__ __device__ __forceinline__ size_t walk_lds_bytes(uint32_t shells, uint32_t table_k)
 {
     return walk_lds_words(shells) * sizeof(unsigned long long)
-        + (table_k ? 16u * (size_t)table_k + 4u * ((size_t)table_k + 1u) : 0u);
+        + (table_k ? 16u * (size_t)table_k + 4u * ((size_t)table_k + 3u) : 0u);
 }
 
 /* Builds the weight table of a binary32 walk in LDS (WeightTable): one
  * thread runs the recurrence W[0] = 1, W[k+1] = RN(W[k] a), divided by 0.1
- * when it falls below the cutoff (the arithmetic of step()), then every
- * thread converts its entries.  Ends with a barrier. */
+ * when it falls below the cutoff (the arithmetic of step()), and takes the
+ * cut threshold h_cut, then every thread converts its entries.  Ends with a
+ * barrier. */
 __device__ __forceinline__ WeightTable build_weight_table(unsigned long long* lds, uint32_t shells, uint32_t K,
                                                           float albedo)
 {
     uint4* tab = (uint4*)(lds + walk_lds_words(shells));
-    float* wseq = (float*)(tab + K);
+    float* wseq = (float*)(tab + K); /* W[0 .. K], then h_cut as two words */
+    uint32_t* hcut = (uint32_t*)(wseq + K + 1);
     if (threadIdx.x == 0) {
         float w = 1.0f;
         for (uint32_t k = 0; k < K; ++k) {
@@ -67,66 +81,78 @@ __device__ __forceinline__ WeightTable build_weight_table(unsigned long long* ld
             w = w2 < 0.001f ? div_tenth(w2) : w2;
         }
         wseq[K] = w;
+        /* T: the least float with RN(T a) >= 0.001f, by bisection over the
+         * bit patterns of (0, 1] (ordered as the floats); none when even
+         * 1 * a falls below, and then every weight does */
+        uint64_t h_cut = ~0ull;
+        if (!(1.0f * albedo < 0.001f)) {
+            uint32_t lo = 0u, hi = __float_as_uint(1.0f); /* RN(lo a) < 0.001f <= RN(hi a) */
+            while (hi - lo > 1u) {
+                const uint32_t mid = lo + (hi - lo) / 2u;
+                if (__uint_as_float(mid) * albedo < 0.001f)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            h_cut = fx_heat(__uint_as_float(hi));
+        }
+        hcut[0] = (uint32_t)h_cut;
+        hcut[1] = (uint32_t)(h_cut >> 32);
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
         const float w = wseq[k];
         const uint64_t h = fx_heat(w), h2 = fx_heat2(w);
-        const bool cut = w * albedo < 0.001f;
-        tab[k] = make_uint4((uint32_t)h, (uint32_t)(h >> 32) | (cut ? 0x80000000u : 0u), (uint32_t)h2,
-                            (uint32_t)(h2 >> 32));
+        tab[k] = make_uint4((uint32_t)h, (uint32_t)(h >> 32), (uint32_t)h2, (uint32_t)(h2 >> 32));
     }
     __syncthreads();
-    return WeightTable{ tab, K, wseq[K] };
+    const uint64_t h_cut = ((uint64_t)hcut[1] << 32) | hcut[0];
+    return WeightTable{ tab, K, wseq[K], h_cut, true };
 }
 
-/* Moves this workgroup's LDS tallies (shell pairs, then the overflow
+/* Moves this workgroup's LDS tallies (inner shell pairs, then the overflow
  * shell's copies, each a (w image, w*w image) pair of words) into the global
  * 128-bit sums.  Safe while other waves of the workgroup keep adding: each
  * entry is taken by an atomic swap with zero, so every addition is counted
  * exactly once. */
-__device__ __forceinline__ void flush_tallies(unsigned long long* lds, unsigned long long* hot,
-                                              unsigned long long* g, uint32_t shells, uint32_t lane)
+__device__ __forceinline__ void flush_tallies(unsigned long long* lds, unsigned long long* g, uint32_t shells,
+                                              uint32_t lane)
 {
-    for (uint32_t s = lane; s < shells; s += 64) {
-        const unsigned long long a = atomicExch(&lds[2 * s], 0ull);
-        const unsigned long long b = atomicExch(&lds[2 * s + 1], 0ull);
+    const uint32_t last = shells - 1;
+    for (uint32_t q = lane; q < last + TMC_WALK_HOT; q += 64) {
+        const unsigned long long a = atomicExch(&lds[2 * q], 0ull);
+        const unsigned long long b = atomicExch(&lds[2 * q + 1], 0ull);
+        const uint32_t s = q < last ? q : last;
         if (a)
             add128(g + 2 * s, a);
         if (b)
             add128(g + 2 * shells + 2 * s, b);
     }
-    if (lane < TMC_WALK_HOT) {
-        const uint32_t last = shells - 1;
-        const unsigned long long a = atomicExch(&hot[2 * lane], 0ull);
-        const unsigned long long b = atomicExch(&hot[2 * lane + 1], 0ull);
-        if (a)
-            add128(g + 2 * last, a);
-        if (b)
-            add128(g + 2 * shells + 2 * last, b);
-    }
 }
 
-/* One absorption event, by photon state type: the reference's FP32 step or
- * one of the precision variants (tmc_step_prec.h). */
-/* VARIANT: the math mode for the binary32 walk, compensation for the
+/* One iteration of every lane of the wave, by photon state type: the
+ * reference's FP32 step or one of the precision variants
+ * (tmc_step_prec.h); returns the lanes whose photon it ends.
+ * VARIANT: the math mode for the binary32 walk, compensation for the
  * weight-from-counts walk, unused for binary64. */
 template <uint32_t SAMPLER, uint32_t VARIANT, bool TABLE>
-__device__ __forceinline__ bool step_of(Photon& p, const Physics& ph, const LaneTally& lt, const WeightTable& wt)
+__device__ __forceinline__ uint64_t step_of(Photon& p, const Physics& ph, const LaneTally& lt, const WeightTable& wt)
 {
-    return step<SAMPLER, VARIANT, TABLE>(p, ph, lt, wt);
+    if (SAMPLER == 0u)
+        return step_wave<SAMPLER, VARIANT, TABLE>(p, ph, lt, wt);
+    return __ballot(!step<SAMPLER, VARIANT, TABLE>(p, ph, lt, wt));
 }
 
 template <uint32_t SAMPLER, uint32_t VARIANT, bool TABLE>
-__device__ __forceinline__ bool step_of(PhotonW& p, const PhysicsW& ph, const LaneTally& lt, const WeightTable&)
+__device__ __forceinline__ uint64_t step_of(PhotonW& p, const PhysicsW& ph, const LaneTally& lt, const WeightTable&)
 {
-    return step<SAMPLER, VARIANT != 0u>(p, ph, lt);
+    return __ballot(!step<SAMPLER, VARIANT != 0u>(p, ph, lt));
 }
 
 template <uint32_t SAMPLER, uint32_t VARIANT, bool TABLE>
-__device__ __forceinline__ bool step_of(Photon64& p, const Physics64& ph, const LaneTally& lt, const WeightTable&)
+__device__ __forceinline__ uint64_t step_of(Photon64& p, const Physics64& ph, const LaneTally& lt, const WeightTable&)
 {
-    return step<SAMPLER>(p, ph, lt);
+    return __ballot(!step<SAMPLER>(p, ph, lt));
 }
 
 /* Whether the next iteration of a live photon is an absorption (only the
@@ -152,17 +178,16 @@ template <class PhotonT, class PhysicsT, uint32_t SAMPLER, uint32_t VARIANT, boo
 __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phys)
 {
     extern __shared__ unsigned long long lds_tally[];
-    unsigned long long* hot = lds_tally + 2 * a.shells;
     for (uint32_t i = threadIdx.x; i < walk_lds_words(a.shells); i += blockDim.x)
         lds_tally[i] = 0;
-    WeightTable wt = { nullptr, 0u, 0.0f };
+    WeightTable wt = { nullptr, 0u, 0.0f, 0ull, false };
     if (TABLE)
         wt = build_weight_table(lds_tally, a.shells, a.table_k, a.albedo); /* ends with a barrier */
     else
         __syncthreads();
 
     const uint32_t lane = __lane_id();
-    const LaneTally lt = { lds_tally, hot + 2 * (lane & (TMC_WALK_HOT - 1)) };
+    LaneTally lt = { lds_tally, tally_fmax(a.shell_max, lane & (TMC_WALK_HOT - 1)), 0ull };
 
     /* wave-uniform pool of claimed photon offsets [pool_next, pool_end) */
     uint64_t pool_next = 0, pool_end = 0;
@@ -170,22 +195,25 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
     uint64_t wave_interactions = 0, wave_truncated = 0;
     uint32_t since_flush = 0;
 
-    /* lane state: one photon (photon.c:11-18).  A lane holds a live photon
-     * while its weight is >= 0 (a live weight can be 0, at albedo 0); -1
-     * marks a lane without one.  Keeping the flag a comparison lets every
-     * ballot of it be one vector compare. */
+    /* lane state: one photon (photon.c:11-18); the lanes without one are
+     * the wave-uniform mask `dead` (scalar registers: a lane's liveness
+     * costs no vector compare, and the step runs under the mask itself) */
     PhotonT ph;
     launch_photon(ph, 0, phys.key0, phys.key1);
-    ph.weight = -1;
-    bool capping = cap_within(ph, a.kcap, a.flush_interval); /* per-step cap checks until the next flush */
+    uint64_t dead = __builtin_amdgcn_read_exec(); /* none has one yet */
+    bool capping = cap_within(ph, false, a.kcap, a.flush_interval); /* per-step cap checks until the next flush */
+    if (TABLE)
+        wt.near_end = table_near_end(ph, false, wt.k_end);
 
     for (;;) {
-        const uint64_t dead = __ballot(ph.weight < 0);
-        if (dead != 0 && !exhausted) {
-            /* the dead lanes take photons from the pool; an empty pool is
-             * refilled with the next chunk first.  Lanes the pool cannot
-             * serve wait for the next step's claim, so every launch indexes
-             * one contiguous range (pool_next + rank). */
+        /* the dead lanes take photons from the pool once there are
+         * TMC_WALK_RELAUNCH_MIN of them (or the whole wave); an empty pool
+         * is refilled with the next chunk first.  Lanes the pool cannot serve
+         * wait for a later claim, so every launch indexes one contiguous
+         * range (pool_next + rank). */
+        /* (a 32-bit count: compared as 64 bits it takes a vector compare) */
+        if ((uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)__popcll(dead)) >= TMC_WALK_RELAUNCH_MIN
+            && !exhausted) {
             if (pool_next == pool_end) {
                 unsigned long long base = 0;
                 if (lane == 0)
@@ -199,29 +227,38 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
                 }
             }
             if (!exhausted) {
-                const uint32_t take = pool_take(pool_next, pool_end, (uint32_t)__popcll(dead));
+                const uint32_t n = (uint32_t)__popcll(dead);
+                const uint32_t take = pool_take(pool_next, pool_end, n);
                 const uint64_t first = a.first_photon + pool_next;
-                if (ph.weight < 0 && mask_rank(dead) < take)
+                const bool gets = __builtin_amdgcn_inverse_ballot_w64(dead) && mask_rank(dead) < take;
+                if (gets)
                     launch_photon(ph, first + mask_rank(dead), phys.key0, phys.key1);
+                /* the usual case, every dead lane served, without a ballot */
+                dead = take == n ? 0ull : dead & ~__ballot(gets);
                 pool_next += take;
             }
         }
-        const uint64_t live = __ballot(ph.weight >= 0);
+        const uint64_t live = __builtin_amdgcn_read_exec() & ~dead;
         if (live == 0)
             break;
-        wave_interactions += (uint64_t)__popcll(SAMPLER == 1u ? __ballot(ph.weight >= 0 && absorbs_of<SAMPLER>(ph))
-                                                              : live);
+        const bool lane_live = __builtin_amdgcn_inverse_ballot_w64(live);
+        wave_interactions +=
+            (uint64_t)__popcll(SAMPLER == 1u ? __ballot(lane_live && absorbs_of<SAMPLER>(ph)) : live);
 
-        if (ph.weight >= 0 && !step_of<SAMPLER, VARIANT, TABLE>(ph, phys, lt, wt))
-            ph.weight = -1;
+        /* every lane steps, its deposits masked by `live`: a lane without a
+         * photon walks a stale one, which a launch overwrites.  Around a
+         * branch on liveness, the lanes the step ends would be a lane mask
+         * merged at the join, whose ballot costs two vector instructions per
+         * step; here it is the step's own compare. */
+        lt.deposit = live;
+        dead |= step_of<SAMPLER, VARIANT, TABLE>(ph, phys, lt, wt) & live;
         /* photons that reached the absorption cap stop; checked only while
          * some photon of the wave can reach it before the next flush */
         if (capping) {
-            const uint64_t capped = __ballot(ph.k >= a.kcap) & __ballot(ph.weight >= 0);
+            const uint64_t capped = __ballot(ph.k >= a.kcap) & live & ~dead;
             if (capped != 0) {
                 wave_truncated += (uint64_t)__popcll(capped);
-                if (ph.k >= a.kcap)
-                    ph.weight = -1;
+                dead |= capped;
             }
         }
 
@@ -229,11 +266,14 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
         since_flush = __builtin_amdgcn_readfirstlane(since_flush + 1u);
         if (since_flush >= a.flush_interval) {
             since_flush = 0;
-            flush_tallies(lds_tally, hot, a.tally, a.shells, lane);
-            capping = cap_within(ph, a.kcap, a.flush_interval);
+            flush_tallies(lds_tally, a.tally, a.shells, lane);
+            capping = cap_within(ph, !__builtin_amdgcn_inverse_ballot_w64(dead), a.kcap, a.flush_interval);
         }
+        /* at most TMC_TABLE_CHECK_STEPS steps apart, flushes included */
+        if (TABLE && since_flush % TMC_TABLE_CHECK_STEPS == 0u)
+            wt.near_end = table_near_end(ph, !__builtin_amdgcn_inverse_ballot_w64(dead), wt.k_end);
     }
-    flush_tallies(lds_tally, hot, a.tally, a.shells, lane);
+    flush_tallies(lds_tally, a.tally, a.shells, lane);
     if (lane == 0) {
         atomicAdd(a.counters + 1, (unsigned long long)wave_interactions);
         if (wave_truncated)
@@ -392,10 +432,13 @@ __global__ void check_short_op_kernel(uint32_t first, uint64_t count, unsigned l
 }
 
 /* Checks shell_index's claim on the squared radii r2 with bit patterns
- * [first, first + count) for one set's spm and shell_max: where the short
- * coordinate is marked sure, its floor and its side of shell_max must be
- * those of RN(RN(sqrt(r2)) * spm).  out[0] += violations, out[1] = min of
- * the violating patterns, out[2] += radii not marked sure. */
+ * [first, first + count) for one set's spm and shell_max, for each of the
+ * overflow copies 0 .. TMC_WALK_HOT - 1 (fmax = shell_max + 1/2 + copy):
+ * where the clamped short coordinate fc is marked sure, floor(fc) must be
+ * floor(fe) for fe = RN(RN(sqrt(r2)) * spm) < shell_max, and a pair from
+ * shell_max to shell_max + copy otherwise.  out[0] += violations (over all
+ * copies), out[1] = min of the violating patterns, out[2] += radii not marked
+ * sure for copy 0. */
 __global__ void check_shell_kernel(float spm, float shell_max, uint32_t first, uint64_t count,
                                    unsigned long long* out)
 {
@@ -403,22 +446,26 @@ __global__ void check_shell_kernel(float spm, float shell_max, uint32_t first, u
     ph.spm = spm;
     ph.shell_max = shell_max;
     set_derived(ph);
+    const uint32_t last = (uint32_t)shell_max;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     unsigned long long bad = 0, lowest = ~0ull, unsure = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
         const uint32_t bits = first + (uint32_t)i;
         const float r2 = __uint_as_float(bits);
-        bool sure;
-        const float fs = shell_coord_short(r2, ph, &sure);
-        if (!sure) {
-            ++unsure;
-            continue;
-        }
         const float fe = sqrt_rn(r2) * spm;
-        const bool in_s = fs < shell_max, in_e = fe < shell_max;
-        if (in_s != in_e || (in_e && (uint32_t)fs != (uint32_t)fe)) {
-            ++bad;
-            lowest = bits < lowest ? bits : lowest;
+        for (uint32_t copy = 0; copy < TMC_WALK_HOT; ++copy) {
+            bool sure;
+            const float fc = shell_coord_short(r2, ph, tally_fmax(shell_max, copy), &sure);
+            if (!sure) {
+                unsure += copy == 0u;
+                continue;
+            }
+            const uint32_t q = (uint32_t)fc;
+            const bool ok = fe < shell_max ? q == (uint32_t)fe : q >= last && q <= last + copy;
+            if (!ok) {
+                ++bad;
+                lowest = bits < lowest ? bits : lowest;
+            }
         }
     }
     if (bad) {

@@ -301,9 +301,11 @@ int tmc_batch_run(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* 
  * batch by timing candidate launches on a reduced copy of it (every set's
  * photon budget divided by `sample_divisor`, at least 64 photons; each
  * candidate the best of three runs).  Starting from *base, a candidate
- * replaces the current choice only when it is at least 3% faster, so the
- * result stays *base unless the sample shows a clear gain.  Writes the
- * choice to *best. */
+ * replaces the current choice only when it is at least 3% faster, and a
+ * final choice other than *base must be 3% faster than *base again on a
+ * copy four times larger (the whole batch when sample_divisor <= 4), so the
+ * result stays *base unless both show a clear gain.  Writes the choice to
+ * *best. */
 int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc_batch_options* base,
                        uint32_t sample_divisor, tmc_batch_options* best);
 

@@ -450,10 +450,14 @@ TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc
     b0.use_graph = 0;
     if (sample_divisor == 0)
         sample_divisor = 1;
-    std::vector<tmc_set> sample(sets, sets + nsets);
-    for (auto& s : sample)
-        if (s.params.photons)
-            s.params.photons = std::max<uint64_t>(64, s.params.photons / sample_divisor);
+    auto reduced = [&](uint32_t divisor) {
+        std::vector<tmc_set> r(sets, sets + nsets);
+        for (auto& s : r)
+            if (s.params.photons)
+                s.params.photons = std::max<uint64_t>(64, s.params.photons / divisor);
+        return r;
+    };
+    const std::vector<tmc_set> sample = reduced(sample_divisor);
 
     /* the automatic chunk work of the full batch on the default shape */
     tmc_batch_plan* probe = nullptr;
@@ -463,34 +467,39 @@ TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc
     const uint64_t auto_cw = probe->chunk_work;
     const double full_work = probe->expected;
     tmc_batch_plan_destroy(probe);
-    double sample_work = 0.0;
-    {
+    /* chunk work is tuned as a multiple of the automatic value, scaled to a
+     * reduced copy so that it sees the same chunk count per wave */
+    auto scale_of = [&](const std::vector<tmc_set>& r, double* scale) -> int {
         tmc_batch_plan* sp = nullptr;
-        rc = tmc_batch_plan_create(sample.data(), nsets, &b0, &sp);
-        if (rc != TMC_OK)
-            return rc;
-        sample_work = sp->expected;
+        const int e = tmc_batch_plan_create(r.data(), nsets, &b0, &sp);
+        if (e != TMC_OK)
+            return e;
+        *scale = full_work > 0 ? sp->expected / full_work : 1.0;
         tmc_batch_plan_destroy(sp);
-    }
-    /* chunk work is tuned as a multiple of the automatic value, scaled to the
-     * sample so that the sample sees the same chunk count per wave */
-    const double scale = full_work > 0 ? sample_work / full_work : 1.0;
+        return TMC_OK;
+    };
+    double scale = 1.0;
+    rc = scale_of(sample, &scale);
+    if (rc != TMC_OK)
+        return rc;
 
-    /* each candidate: the best of three runs after a warm-up run */
-    auto time_of = [&](const tmc_batch_options& o, double mult) -> double {
+    /* each candidate: the best of `reps` runs after a warm-up run */
+    auto time_on = [&](const std::vector<tmc_set>& r, double sc, const tmc_batch_options& o, double mult,
+                       int reps) -> double {
         tmc_batch_options t = o;
-        t.chunk_work = (uint64_t)std::max(4096.0, mult * (double)auto_cw * scale);
+        t.chunk_work = (uint64_t)std::max(4096.0, mult * (double)auto_cw * sc);
         tmc_batch_plan* plan = nullptr;
-        if (tmc_batch_plan_create(sample.data(), nsets, &t, &plan) != TMC_OK)
+        if (tmc_batch_plan_create(r.data(), nsets, &t, &plan) != TMC_OK)
             return 1e300;
         tmc_batch_stats st;
         double best_ms = 1e300;
-        for (int rep = 0; rep < 4; ++rep) /* first run warms the code object */
+        for (int rep = 0; rep <= reps; ++rep) /* first run warms the code object */
             if (tmc_batch_plan_run(plan, nullptr, &st) == TMC_OK && rep > 0)
                 best_ms = std::min(best_ms, st.kernel_ms);
         tmc_batch_plan_destroy(plan);
         return best_ms;
     };
+    auto time_of = [&](const tmc_batch_options& o, double mult) { return time_on(sample, scale, o, mult, 3); };
     /* a candidate replaces the current options only when it is faster by
      * kMargin on the sample: the sample's shorter queues favour shapes that
      * do not carry over to the full batch, and the automatic options already
@@ -536,6 +545,26 @@ TMC_EXPORT int tmc_batch_autotune(const tmc_set* sets, uint32_t nsets, const tmc
     }
     if (cur_ms >= 1e300)
         return TMC_E_HIP;
+    /* a choice other than *base must also win on a copy four times larger
+     * (the whole batch for divisors up to 4): on the 4096-set sweep the
+     * 1/16 sample once picked 4x chunk work and 6 waves per SIMD, 2.4% slower
+     * than the defaults on the full batch */
+    const bool changed = cur_mult != 1.0 || cur.block != b0.block || cur.waves_per_simd != b0.waves_per_simd
+        || cur.copies != b0.copies;
+    if (changed) {
+        const uint32_t div2 = sample_divisor / 4u > 1u ? sample_divisor / 4u : 1u;
+        const std::vector<tmc_set> confirm = div2 > 1u ? reduced(div2) : std::vector<tmc_set>(sets, sets + nsets);
+        double sc2 = 1.0;
+        rc = scale_of(confirm, &sc2);
+        if (rc != TMC_OK)
+            return rc;
+        const double base_ms = time_on(confirm, sc2, b0, 1.0, 2);
+        const double pick_ms = time_on(confirm, sc2, cur, cur_mult, 2);
+        if (!(pick_ms < kMargin * base_ms)) {
+            cur = b0;
+            cur_mult = 1.0;
+        }
+    }
     *best = cur;
     /* unchanged chunk work stays automatic (the plan derives it per pass) */
     best->chunk_work = cur_mult == 1.0 ? b0.chunk_work : (uint64_t)std::max(4096.0, cur_mult * (double)auto_cw);

@@ -83,10 +83,14 @@ struct LaneTally {
 /* fmax of a lane for copy `copy` of the overflow shell. */
 __device__ __forceinline__ float tally_fmax(float shell_max, uint32_t copy) { return shell_max + 0.5f + (float)copy; }
 
-/* Adds one deposit's images to pair q (lanes in lt.deposit only). */
+/* Adds one deposit's images to pair q: MASKED, for the lanes in lt.deposit
+ * only; else for every active lane (a walk that steps only the lanes holding
+ * a photon: there the mask would only split the step into blocks the
+ * compiler schedules apart). */
+template <bool MASKED = true>
 __device__ __forceinline__ void tally_add(const LaneTally& lt, uint32_t q, unsigned long long a, unsigned long long b)
 {
-    if (__builtin_amdgcn_inverse_ballot_w64(lt.deposit)) {
+    if (!MASKED || __builtin_amdgcn_inverse_ballot_w64(lt.deposit)) {
         unsigned long long* p = lt.base + 2u * q;
         atomicAdd(p, a);
         atomicAdd(p + 1, b);
@@ -241,7 +245,7 @@ struct StepEnd {
     uint32_t x3;
 };
 
-template <uint32_t SAMPLER, uint32_t MATH, bool TABLE>
+template <uint32_t SAMPLER, uint32_t MATH, bool TABLE, bool MASKED>
 __device__ __forceinline__ StepEnd step_core(Photon& p, const Physics& ph, const LaneTally& lt, const WeightTable& wt)
 {
     const bool fresh = absorbs<SAMPLER>(p);
@@ -286,9 +290,9 @@ __device__ __forceinline__ StepEnd step_core(Photon& p, const Physics& ph, const
                 }
             }
             cut = h < wt.h_cut; /* = RN(w a) < 0.001f for the deposited weight w */
-            tally_add(lt, q, h, h2);
+            tally_add<MASKED>(lt, q, h, h2);
         } else {
-            tally_add(lt, q, fx_heat(p.weight), fx_heat2(p.weight));
+            tally_add<MASKED>(lt, q, fx_heat(p.weight), fx_heat2(p.weight));
             p.weight = p.weight * ph.albedo;
             cut = p.weight < 0.001f;
         }
@@ -342,12 +346,13 @@ __device__ __forceinline__ StepEnd step_core(Photon& p, const Physics& ph, const
 }
 
 /* One iteration of p (step_core), then the roulette (photon.c:45-49).
- * Returns false when the roulette kills the photon. */
-template <uint32_t SAMPLER, uint32_t MATH, bool TABLE = false>
+ * Returns false when the roulette kills the photon.  MASKED: deposits for
+ * the lanes in lt.deposit only (tally_add). */
+template <uint32_t SAMPLER, uint32_t MATH, bool TABLE = false, bool MASKED = false>
 __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTally& lt,
                                      const WeightTable& wt = WeightTable{ nullptr, 0u, 0.0f, 0ull, false })
 {
-    const StepEnd e = step_core<SAMPLER, MATH, TABLE>(p, ph, lt, wt);
+    const StepEnd e = step_core<SAMPLER, MATH, TABLE, MASKED>(p, ph, lt, wt);
     if (e.cut) {
         if (e.x3 >= TMC_ROULETTE_KILL_R) /* u > 0.1f */
             return false;
@@ -365,10 +370,13 @@ __device__ __forceinline__ bool step(Photon& p, const Physics& ph, const LaneTal
 template <uint32_t SAMPLER, uint32_t MATH, bool TABLE>
 __device__ __forceinline__ uint64_t step_wave(Photon& p, const Physics& ph, const LaneTally& lt, const WeightTable& wt)
 {
-    const StepEnd e = step_core<SAMPLER, MATH, TABLE>(p, ph, lt, wt);
+    const StepEnd e = step_core<SAMPLER, MATH, TABLE, true>(p, ph, lt, wt);
+    /* (taken before the branch below, across which the compiler would
+     * rebuild the cut flag's ballot from a merged lane mask) */
+    const uint64_t ends = __ballot(e.cut) & __ballot(e.x3 >= TMC_ROULETTE_KILL_R); /* u > 0.1f */
     if (!TABLE && e.cut) /* (a photon the roulette kills drops its weight) */
         p.weight = div_tenth(p.weight); /* = p.weight / 0.1f */
-    return __ballot(e.cut) & __ballot(e.x3 >= TMC_ROULETTE_KILL_R); /* u > 0.1f */
+    return ends;
 }
 
 } // namespace tmc

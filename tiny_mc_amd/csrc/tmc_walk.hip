@@ -140,7 +140,7 @@ __device__ __forceinline__ uint64_t step_of(Photon& p, const Physics& ph, const 
 {
     if (SAMPLER == 0u)
         return step_wave<SAMPLER, VARIANT, TABLE>(p, ph, lt, wt);
-    return __ballot(!step<SAMPLER, VARIANT, TABLE>(p, ph, lt, wt));
+    return __ballot(!step<SAMPLER, VARIANT, TABLE, true>(p, ph, lt, wt));
 }
 
 template <uint32_t SAMPLER, uint32_t VARIANT, bool TABLE>

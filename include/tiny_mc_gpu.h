@@ -216,11 +216,15 @@ int tiny_mc_gpu_check_short_op(uint32_t op, uint32_t first_bits, uint64_t count,
 
 /* Diagnostic: the binary32 walk takes a deposit's shell from the hardware
  * square root where that is certain to give the shell of the correctly
- * rounded one (photon.c:26-29).  Checks that claim for the physics p on
- * every squared radius whose bit pattern lies in [first_bits, first_bits +
- * count): violations (radii marked certain whose shell or overflow side
- * differs) with the lowest such pattern, and the number of radii not marked
- * certain (the walk recomputes those with the IEEE square root). */
+ * rounded one (photon.c:26-29), its coordinate clamped to a lane's bound
+ * past the last shell, which picks that lane's copy of the overflow shell.
+ * Checks that claim for the physics p on every squared radius whose bit
+ * pattern lies in [first_bits, first_bits + count) and for each of the 32
+ * copies: violations (radii marked certain whose inner shell differs, or
+ * that land on the wrong side of the overflow boundary or outside the
+ * copies) with the lowest such pattern, and the number of radii not marked
+ * certain for copy 0 (the walk recomputes those with the IEEE square
+ * root). */
 int tiny_mc_gpu_check_shell_index(const tmc_params* p, uint32_t first_bits, uint64_t count, uint64_t* violations,
                                   uint32_t* first_violation, uint64_t* unsure);
 

@@ -35,7 +35,9 @@ def test_restatement_matches_reference_table(orc, name):
 
 
 def test_default_fixture_hash():
-    """The seed-12345 table is the one whose hash the survey recorded."""
+    """The seed-12345 table is the unmodified reference's output for
+    SEED=12345, PHOTONS=32768, default params.h (tiny_mc.c:37-66 format),
+    pinned by its sha256 as taken when the reference was first built."""
     m = _manifest()
     assert (m["files"]["ref_default_s12345_n32768.txt"]["sha256"]
             == "10dcd4cb018ed664be4b74cf859454bda24dc76e7a709c5a4d7ed7da3c382166")

@@ -45,7 +45,10 @@ namespace tmc {
 /* Copies of the overflow shell's sums in each copy of a wave's slot
  * (lane & 15 picks one), after the inner shells (LaneTally).  Shells and
  * copies alike are (w image, w*w image) pairs of adjacent words. */
+#ifndef TMC_SLOT_HOT
 #define TMC_SLOT_HOT 16u
+#endif
+static_assert(TMC_SLOT_HOT <= TMC_CHECK_HOT, "the shell check covers every overflow copy");
 
 /* 64-bit words of one copy of a slot: room for the inner shells of a set of
  * up to shells_max shells and its overflow copies. */

@@ -80,6 +80,10 @@ struct LaneTally {
     uint64_t deposit;
 };
 
+/* Overflow copies tiny_mc_gpu_check_shell_index covers (copies 0 .. 31):
+ * at least as many as either walk kernel uses. */
+#define TMC_CHECK_HOT 32u
+
 /* fmax of a lane for copy `copy` of the overflow shell. */
 __device__ __forceinline__ float tally_fmax(float shell_max, uint32_t copy) { return shell_max + 0.5f + (float)copy; }
 

@@ -33,9 +33,15 @@
 
 namespace tmc {
 
-/* Copies of the overflow shell's sums per workgroup (lane & 31 picks one):
- * 32 consecutive pairs of words. */
-#define TMC_WALK_HOT 32u
+/* Copies of the overflow shell's sums per workgroup (lane & 7 picks one's
+ * bound, LaneTally).  Measured per launch (2^30 photons at default physics,
+ * 2^22 at mu_a = 0.01, mu_s = 100 over 401 x 10 um, 2^26 at albedo 0.98 over
+ * 201 x 20 um): 32 copies 136.7, 654, 46.2 ms; 16: 134.6, 657, 44.2; 8:
+ * 134.0, 656, 44.1; 4: 135.6, 654, 44.1; 2: 136.1, 1077, 66.5. */
+#ifndef TMC_WALK_HOT
+#define TMC_WALK_HOT 8u
+#endif
+static_assert(TMC_WALK_HOT <= TMC_CHECK_HOT, "the shell check covers every overflow copy");
 
 /* Dead lanes a wave gathers before it launches photons into them: a launch
  * costs the wave about fifteen vector instructions whatever the number of
@@ -140,6 +146,8 @@ __device__ __forceinline__ uint64_t step_of(Photon& p, const Physics& ph, const 
 {
     if (SAMPLER == 0u)
         return step_wave<SAMPLER, VARIANT, TABLE>(p, ph, lt, wt);
+    /* (the rejection sampler's pending attempts merge its cut flag across a
+     * branch: step_wave measured 1.5% slower for it) */
     return __ballot(!step<SAMPLER, VARIANT, TABLE, true>(p, ph, lt, wt));
 }
 
@@ -433,7 +441,7 @@ __global__ void check_short_op_kernel(uint32_t first, uint64_t count, unsigned l
 
 /* Checks shell_index's claim on the squared radii r2 with bit patterns
  * [first, first + count) for one set's spm and shell_max, for each of the
- * overflow copies 0 .. TMC_WALK_HOT - 1 (fmax = shell_max + 1/2 + copy):
+ * overflow copies 0 .. TMC_CHECK_HOT - 1 (fmax = shell_max + 1/2 + copy):
  * where the clamped short coordinate fc is marked sure, floor(fc) must be
  * floor(fe) for fe = RN(RN(sqrt(r2)) * spm) < shell_max, and a pair from
  * shell_max to shell_max + copy otherwise.  out[0] += violations (over all
@@ -453,7 +461,7 @@ __global__ void check_shell_kernel(float spm, float shell_max, uint32_t first, u
         const uint32_t bits = first + (uint32_t)i;
         const float r2 = __uint_as_float(bits);
         const float fe = sqrt_rn(r2) * spm;
-        for (uint32_t copy = 0; copy < TMC_WALK_HOT; ++copy) {
+        for (uint32_t copy = 0; copy < TMC_CHECK_HOT; ++copy) {
             bool sure;
             const float fc = shell_coord_short(r2, ph, tally_fmax(shell_max, copy), &sure);
             if (!sure) {

@@ -51,6 +51,8 @@ static_assert(TMC_WALK_HOT <= TMC_CHECK_HOT, "the shell check covers every overf
 #ifndef TMC_WALK_RELAUNCH_MIN
 #define TMC_WALK_RELAUNCH_MIN 4
 #endif
+/* a wave whose lanes are all dead must launch, or it would leave the walk */
+static_assert(TMC_WALK_RELAUNCH_MIN >= 1 && TMC_WALK_RELAUNCH_MIN <= 64, "a drained wave must relaunch");
 
 /* LDS words of one workgroup: the (w, w*w) sum pairs of the inner shells,
  * then the overflow shell's copies (LaneTally; a multiple of 2 words: the
@@ -215,8 +217,8 @@ __device__ __forceinline__ void walk_body(const WalkArgs& a, const PhysicsT& phy
 
     for (;;) {
         /* the dead lanes take photons from the pool once there are
-         * TMC_WALK_RELAUNCH_MIN of them (or the whole wave); an empty pool
-         * is refilled with the next chunk first.  Lanes the pool cannot serve
+         * TMC_WALK_RELAUNCH_MIN of them (at most 64, so a drained wave always
+         * does); an empty pool is refilled with the next chunk first.  Lanes the pool cannot serve
          * wait for a later claim, so every launch indexes one contiguous
          * range (pool_next + rank). */
         /* (a 32-bit count: compared as 64 bits it takes a vector compare) */

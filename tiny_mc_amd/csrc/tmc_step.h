@@ -247,6 +247,9 @@ __device__ __forceinline__ bool absorbs(const Photon& p)
 struct StepEnd {
     bool cut;
     uint32_t x3;
+    /* with the table and the analytic sampler: the wave's ballot of cut,
+     * taken before the deposit (below) */
+    uint64_t cut_lanes;
 };
 
 template <uint32_t SAMPLER, uint32_t MATH, bool TABLE, bool MASKED>
@@ -267,6 +270,7 @@ __device__ __forceinline__ StepEnd step_core(Photon& p, const Physics& ph, const
         r = tmc_philox4x32_10_block0_ks(p.k, p.pp, ph.keys); /* block (k, 0) */
 
     bool cut = false; /* the weight after this absorption is below the cutoff */
+    uint64_t cut_lanes = 0;
     if (fresh) {
         /* move (photon.c:21-24) */
         const float t = step_length<MATH>(r.x0);
@@ -294,6 +298,12 @@ __device__ __forceinline__ StepEnd step_core(Photon& p, const Physics& ph, const
                 }
             }
             cut = h < wt.h_cut; /* = RN(w a) < 0.001f for the deposited weight w */
+            /* the ballot here, ahead of the masked deposit: with the compare
+             * left for after it, the compiler waited at the join for the
+             * table read and, with it, for both LDS atomics (its counter
+             * is in order) */
+            if (SAMPLER == 0u && MASKED)
+                cut_lanes = __ballot(cut);
             tally_add<MASKED>(lt, q, h, h2);
         } else {
             tally_add<MASKED>(lt, q, fx_heat(p.weight), fx_heat2(p.weight));
@@ -325,7 +335,7 @@ __device__ __forceinline__ StepEnd step_core(Photon& p, const Physics& ph, const
             } else {
                 p.att = (p.att & 0x80000000u) | att;
             }
-            return StepEnd{ false, 0u };
+            return StepEnd{ false, 0u, 0ull };
         }
         if (!fresh) {
             cut = (p.att & 0x80000000u) != 0u;
@@ -346,7 +356,7 @@ __device__ __forceinline__ StepEnd step_core(Photon& p, const Physics& ph, const
         p.w = s * sn;
     }
     ++p.k;
-    return StepEnd{ cut, x3 };
+    return StepEnd{ cut, x3, cut_lanes };
 }
 
 /* One iteration of p (step_core), then the roulette (photon.c:45-49).
@@ -377,7 +387,8 @@ __device__ __forceinline__ uint64_t step_wave(Photon& p, const Physics& ph, cons
     const StepEnd e = step_core<SAMPLER, MATH, TABLE, true>(p, ph, lt, wt);
     /* (taken before the branch below, across which the compiler would
      * rebuild the cut flag's ballot from a merged lane mask) */
-    const uint64_t ends = __ballot(e.cut) & __ballot(e.x3 >= TMC_ROULETTE_KILL_R); /* u > 0.1f */
+    const uint64_t cut_lanes = TABLE && SAMPLER == 0u ? e.cut_lanes : __ballot(e.cut);
+    const uint64_t ends = cut_lanes & __ballot(e.x3 >= TMC_ROULETTE_KILL_R); /* u > 0.1f */
     if (!TABLE && e.cut) /* (a photon the roulette kills drops its weight) */
         p.weight = div_tenth(p.weight); /* = p.weight / 0.1f */
     return ends;

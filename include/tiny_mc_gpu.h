@@ -347,8 +347,9 @@ typedef struct tmc_serve_options {
     uint32_t math; /* TMC_MATH_* */
     uint32_t max_chunk_scale; /* under load (unpublished work beyond two chunks per resident wave)
                                  chunk_work and max_photons_per_lane grow with the backlog, up to
-                                 this factor (fewer wave drains per absorption); 1 = fixed chunks,
-                                 0 = 128, <= 1024 */
+                                 this factor (fewer wave drains per absorption); one request's
+                                 work counts up to four chunks per wave, so a request alone grows
+                                 its chunks at most 2x; 1 = fixed chunks, 0 = 128, <= 1024 */
     uint32_t reserve_per_cu; /* workgroup slots per compute unit the resident kernel leaves free
                                 for other work on the device; 0 = none (the whole device
                                 serves); 2 lets single runs and batches run beside the server */

@@ -51,6 +51,37 @@ def test_product_header_on_host(orc, tmp_path):
         assert tuple(orc.philox(r[:4], r[4:6])) == tuple(r[6:])
 
 
+def test_photon_shortcuts_on_host(tmp_path):
+    """The per-photon shortcuts of the product header (tmc_philox_photon_init
+    with tmc_philox4x32_10_block0, _block0_ks, _block0_r2 and _blockb, which take the
+    photon's part of rounds 1-2 precomputed) against the full generator
+    (pinned above by the vectors), for 20000 pseudo-random (k, b, photon,
+    key), b in 0..3: a dropped key, word or round would differ."""
+    src = tmp_path / "short.c"
+    src.write_text(
+        '#include <stdio.h>\n#include "tmc_philox.h"\n'
+        "static int same(tmc_u32x4 a, tmc_u32x4 b) { return a.x0 == b.x0 && a.x1 == b.x1 && a.x2 == b.x2 && a.x3 == b.x3; }\n"
+        "int main(void){ unsigned s = 7, bad = 0;\n"
+        "  for (int i = 0; i < 20000; ++i) { unsigned c[6];\n"
+        "    for (int j = 0; j < 6; ++j) { s = s * 1664525u + 1013904223u; c[j] = s; }\n"
+        "    const unsigned k = c[0], b = c[1] & 3u, plo = c[2], phi = c[3], k0 = c[4], k1 = c[5];\n"
+        "    const tmc_philox_photon pp = tmc_philox_photon_init(plo, phi, k0, k1);\n"
+        "    const tmc_philox_keys ks = tmc_philox_key_schedule(k0, k1);\n"
+        "    const tmc_u32x4 want0 = tmc_philox4x32_10(k, 0u, plo, phi, k0, k1);\n"
+        "    const tmc_u32x4 wantb = tmc_philox4x32_10(k, b, plo, phi, k0, k1);\n"
+        "    bad += !same(tmc_philox4x32_10_block0(k, pp, k0, k1), want0);\n"
+        "    bad += !same(tmc_philox4x32_10_block0_ks(k, pp, ks), want0);\n"
+        "    bad += !same(tmc_philox4x32_10_block0_r2(k, pp, ks), want0);\n"
+        "    bad += !same(tmc_philox4x32_10_blockb(k, b, pp, &ks, k0, k1), wantb);\n"
+        "  }\n"
+        "  printf(\"%u\\n\", bad); return 0; }\n")
+    exe = tmp_path / "short"
+    subprocess.run(["gcc", "-std=c11", "-O2", "-Wall", "-I", os.path.join(ROOT, "tiny_mc_amd", "csrc"), "-o",
+                    str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert out.strip() == "0"
+
+
 @pytest.mark.gpu
 def test_device_philox_kat(tmc, orc):
     rng = np.random.default_rng(7)

@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             /* the loop has one back-edge (no continue): with several, the
              * compiler copied the photon state in and out of the lane-masked
              * step every iteration */
-            if (ph.weight >= 0 && !step<SAMPLER, MATH>(ph, st.phys, lt))
+            if (ph.weight >= 0 && !step<SAMPLER, MATH, false, false, true>(ph, st.phys, lt))
                 ph.weight = -1;
             /* photons that reached the absorption cap stop; checked only while
              * some photon of the wave can reach it before the next flush */

@@ -67,20 +67,27 @@ TMC_HD tmc_u32x4 tmc_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32
     return r;
 }
 
-/* The part of round 1 of block (k, 0) that depends only on the photon
+/* The part of rounds 1-2 of block (k, 0) that depends only on the photon
  * (counter words 2-3) and the key: for counter (k, 0, c2, c3) round 1 gives
  *   c0' = hi(c2 * M1) ^ 0 ^ key0,  c1' = lo(c2 * M1),
  *   c2' = hi(k * M0) ^ c3 ^ key1,  c3' = lo(k * M0),
  * so a walk computes c0', c1' and c3 ^ key1 once per photon and only the
- * product k * M0 per absorption. */
+ * product k * M0 per absorption.  Round 2 multiplies c0' by M0, again a
+ * photon constant: its high word (with round 2's key1 folded in, c2'' =
+ * hi(c0' M0) ^ c3' ^ (key1 + W1)) and its low word (c3'') are kept too, for
+ * block 0 (tmc_philox4x32_10_block0_r2); block b > 0 changes c0'. */
 typedef struct tmc_philox_photon {
     uint32_t c0r1, c1r1, c3k1;
+    uint32_t c2r2k, c3r2;
 } tmc_philox_photon;
 
 TMC_HD tmc_philox_photon tmc_philox_photon_init(uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
 {
     const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
-    tmc_philox_photon pp = { (uint32_t)(p1 >> 32) ^ k0, (uint32_t)p1, c3 ^ k1 };
+    const uint32_t c0r1 = (uint32_t)(p1 >> 32) ^ k0;
+    const uint64_t p0 = (uint64_t)c0r1 * TMC_PHILOX_M0;
+    tmc_philox_photon pp = { c0r1, (uint32_t)p1, c3 ^ k1, (uint32_t)(p0 >> 32) ^ (k1 + TMC_PHILOX_W1),
+                             (uint32_t)p0 };
     return pp;
 }
 
@@ -136,6 +143,33 @@ TMC_HD tmc_u32x4 tmc_philox4x32_10_block0_ks(uint32_t k, tmc_philox_photon pp, t
 #pragma unroll
 #endif
     for (int round = 1; round < 10; ++round) {
+        const uint64_t p0 = (uint64_t)c0 * TMC_PHILOX_M0;
+        const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
+        const uint32_t n0 = tmc_xor3((uint32_t)(p1 >> 32), c1, ks.k0[round]);
+        const uint32_t n2 = tmc_xor3((uint32_t)(p0 >> 32), c3, ks.k1[round]);
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    tmc_u32x4 r = { c0, c1, c2, c3 };
+    return r;
+}
+
+/* tmc_philox4x32_10_block0_ks with round 2's product of the photon constant
+ * c0' taken from pp: one multiplication fewer per block (two more words per
+ * photon). */
+TMC_HD tmc_u32x4 tmc_philox4x32_10_block0_r2(uint32_t k, tmc_philox_photon pp, tmc_philox_keys ks)
+{
+    const uint64_t q0 = (uint64_t)k * TMC_PHILOX_M0;
+    /* after round 1: (c0r1, c1r1, hi(q0) ^ c3k1, lo(q0)); round 2: */
+    const uint64_t p1 = (uint64_t)((uint32_t)(q0 >> 32) ^ pp.c3k1) * TMC_PHILOX_M1;
+    uint32_t c0 = tmc_xor3((uint32_t)(p1 >> 32), pp.c1r1, ks.k0[1]), c1 = (uint32_t)p1,
+             c2 = pp.c2r2k ^ (uint32_t)q0, c3 = pp.c3r2;
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int round = 2; round < 10; ++round) {
         const uint64_t p0 = (uint64_t)c0 * TMC_PHILOX_M0;
         const uint64_t p1 = (uint64_t)c2 * TMC_PHILOX_M1;
         const uint32_t n0 = tmc_xor3((uint32_t)(p1 >> 32), c1, ks.k0[round]);

@@ -513,21 +513,25 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
                 flags &= ~PENDING;
             }
         }
+        /* the lanes holding a photon: those not dead, and the dead ones that
+         * the refill below gives one (all of them unless the pool runs
+         * short), kept in scalar registers: the step runs under this mask
+         * with no vector compare of its own */
+        uint64_t live = ~dead;
         /* the dead lanes take photons from the pool once there are
          * TMC_SLOT_RELAUNCH_MIN of them or the wave has drained */
-        if (dead != 0
-            && ((uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)__popcll(dead)) >= TMC_SLOT_RELAUNCH_MIN
-                || dead == ~0ull)) {
+        const uint32_t n_dead = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)__popcll(dead));
+        if (dead != 0 && (n_dead >= TMC_SLOT_RELAUNCH_MIN || dead == ~0ull)) {
             if (pool_next != pool_end) {
-                const uint32_t take = pool_take(pool_next, pool_end, (uint32_t)__popcll(dead));
+                const uint32_t take = pool_take(pool_next, pool_end, n_dead);
                 const uint64_t first = st.first_photon + pool_next;
                 if (ph.weight < 0 && mask_rank(dead) < take)
                     launch_photon(ph, first + mask_rank(dead), st.phys.key0, st.phys.key1);
                 pool_next += take;
                 launched += take;
+                live = take == n_dead ? ~0ull : __ballot(ph.weight >= 0);
             }
         }
-        const uint64_t live = __ballot(ph.weight >= 0);
         /* the loop ends with no photon left and no chunk to come (a loop
          * condition rather than a break: with a break, or with this
          * assignment in an else branch of the step, the compiler tested the
@@ -541,7 +545,7 @@ __global__ void __launch_bounds__(1024) walk_multi(BatchArgs a)
             /* the loop has one back-edge (no continue): with several, the
              * compiler copied the photon state in and out of the lane-masked
              * step every iteration */
-            if (ph.weight >= 0 && !step<SAMPLER, MATH, false, false, true>(ph, st.phys, lt))
+            if (__builtin_amdgcn_inverse_ballot_w64(live) && !step<SAMPLER, MATH, false, false, true>(ph, st.phys, lt))
                 ph.weight = -1;
             /* photons that reached the absorption cap stop; checked only while
              * some photon of the wave can reach it before the next flush */

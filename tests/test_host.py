@@ -80,6 +80,13 @@ def test_serving_requests_distribution(tmc):
     assert abs(np.median(photons) / 26008 - 1.0) < 0.05
     assert len({s for _, s in reqs}) == len(reqs)
     assert tmc.serve_load.mean_request_work(reqs[:100]) > 0
+    # the sweep's physics: mu_a log-uniform on [0.05, 10], mu_s on [10, 200], four shell counts
+    mu_a = np.array([q.mu_a for q, _ in reqs])
+    assert mu_a.min() >= 0.05 * (1 - 1e-6) and mu_a.max() <= 10.0 * (1 + 1e-6)
+    assert abs(np.median(np.log(mu_a)) - 0.5 * (np.log(0.05) + np.log(10.0))) < 0.05
+    assert {q.shells for q, _ in reqs} == set(tmc.sweep.SHELL_CHOICES)
+    # request i depends on (seed, i) only: a shorter list is a prefix
+    assert reqs[:1234] == tmc.serve_load.draw_requests(1234, seed=3)
 
 
 def test_batch_tally_layout(tmc):

@@ -10,7 +10,7 @@ import time
 import numpy as np
 
 from ._lib import Params, expected_interactions
-from .sweep import draw_physics
+from .sweep import SHELL_CHOICES
 
 X_M = 2**14
 PHOTONS_MAX = 2**24
@@ -18,10 +18,18 @@ ALPHA = 1.5
 
 
 def draw_requests(n, seed=0):
-    """n requests: [(Params, seed)]."""
-    rng = np.random.default_rng(seed)
-    mu_a, mu_s, shells, width = draw_physics(rng, n)
-    photons = np.clip(X_M / rng.uniform(0.0, 1.0, n) ** (1.0 / ALPHA), X_M, PHOTONS_MAX).astype(np.int64)
+    """n requests: [(Params, seed)].  Request i is a function of (seed, i)
+    alone: five uniforms per request, drawn row by row, so the first m of n
+    requests are the m requests (an offered rate that differs slightly
+    between runs keeps the same list up to its length)."""
+    u = np.random.default_rng(seed).uniform(0.0, 1.0, (n, 5))
+    mu_a = np.exp(np.log(0.05) + u[:, 0] * (np.log(10.0) - np.log(0.05)))
+    mu_s = np.exp(np.log(10.0) + u[:, 1] * (np.log(200.0) - np.log(10.0)))
+    shells = np.asarray(SHELL_CHOICES)[np.minimum((u[:, 2] * len(SHELL_CHOICES)).astype(np.int64),
+                                                  len(SHELL_CHOICES) - 1)]
+    width = 10.0 + u[:, 3] * 90.0
+    # Pareto(ALPHA, X_M) by inversion of 1 - u in (0, 1]
+    photons = np.clip(X_M / (1.0 - u[:, 4]) ** (1.0 / ALPHA), X_M, PHOTONS_MAX).astype(np.int64)
     return [(Params(float(np.float32(mu_a[i])), float(np.float32(mu_s[i])), int(shells[i]),
                     float(np.float32(width[i])), int(photons[i])), 1_000_000 + i) for i in range(n)]
 

@@ -7,6 +7,7 @@
 // chunk, chunk sizes within the 32-bit fields, a set's chunks in the pass
 // whose LDS slots hold its shells; arena blocks disjoint and first fit).
 // Test infrastructure; prints "plan_check: ok" at the end.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -273,6 +274,59 @@ void check_chunk_photons()
     }
 }
 
+/* The server's chunk sizing: the scale is 1 up to the threshold
+ * (chunks_per_wave base chunks per wave), the backlog over the threshold
+ * beyond, at most max_scale, and monotone; one request's share is capped,
+ * so a request alone raises the scale to at most cap / threshold while many
+ * reach max_scale; and publishing a request chunk by chunk as the server
+ * does, each step taking share(before) - share(after) off the backlog,
+ * brings the backlog back to zero within rounding once the request is
+ * published (share(0) = 0, no drift over many chunks). */
+void check_chunk_scale()
+{
+    for (int t = 0; t < 20000; ++t) {
+        const double base = (double)(1 + below(1u << 20));
+        const uint32_t waves = 1 + (uint32_t)below(1u << 14);
+        const double cpw = 1.0 + (double)below(4);
+        const uint32_t max_scale = 1 + (uint32_t)below(1024);
+        const double thr = cpw * base * (double)waves;
+        const double b = thr * (double)below(1u << 12) / 256.0;
+        const double sc = tmc::serve_chunk_scale(b, base, waves, cpw, max_scale);
+        CHECK(sc >= 1.0 && sc <= (double)max_scale);
+        if (b <= thr)
+            CHECK(sc == 1.0);
+        else if (b / thr <= (double)max_scale)
+            CHECK(sc == b / thr);
+        else
+            CHECK(sc == (double)max_scale);
+        CHECK(tmc::serve_chunk_scale(b * 1.5 + 1.0, base, waves, cpw, max_scale) >= sc);
+    }
+    const double base = 131072.0, cpw = 2.0;
+    const uint32_t waves = 6144, max_scale = 128;
+    const double thr = cpw * base * waves, cap = 2.0 * thr;
+    /* one request of 2^24 photons at 5,750 absorptions each (1e11) alone */
+    const double one = tmc::backlog_share(1ull << 24, 5750.0, cap);
+    CHECK(one == cap);
+    CHECK(tmc::serve_chunk_scale(one, base, waves, cpw, max_scale) == 2.0);
+    CHECK(tmc::backlog_share(1000, 75.5, cap) == 1000 * 75.5);
+    /* 200 such requests queued reach the largest scale */
+    CHECK(tmc::serve_chunk_scale(200 * one, base, waves, cpw, max_scale) == (double)max_scale);
+    for (int t = 0; t < 2000; ++t) {
+        const uint64_t N = 1 + below(below(2) ? 1ull << 24 : 4096);
+        const double L = 1.0 + (double)below(20000) / 3.0;
+        const double c = (double)(1 + below(1ull << 34));
+        double backlog = tmc::backlog_share(N, L, c);
+        const double added = backlog;
+        uint64_t next = 0;
+        while (next < N) {
+            const uint64_t cnt = std::min<uint64_t>(N - next, 1 + below(1 + N / 8));
+            backlog -= tmc::backlog_share(N - next, L, c) - tmc::backlog_share(N - next - cnt, L, c);
+            next += cnt;
+        }
+        CHECK(std::fabs(backlog) <= 1e-9 * added + 1e-6);
+    }
+}
+
 } // namespace
 
 int main()
@@ -281,6 +335,7 @@ int main()
     check_plans();
     check_arena();
     check_chunk_photons();
+    check_chunk_scale();
     std::puts("plan_check: ok");
     return 0;
 }

@@ -272,3 +272,14 @@ uint64_t tmc::chunk_photons(double work, double L, uint64_t max_per_lane, uint64
     per_lane = std::min(std::max(per_lane, 1.0), (double)max_per_lane);
     return std::max<uint64_t>(1, std::min<uint64_t>(left, 64u * (uint64_t)per_lane));
 }
+
+double tmc::backlog_share(uint64_t photons, double L, double cap)
+{
+    return std::min((double)photons * L, cap);
+}
+
+double tmc::serve_chunk_scale(double backlog, double base_work, uint32_t waves, double chunks_per_wave,
+                              uint32_t max_scale)
+{
+    return std::min(std::max(1.0, backlog / (chunks_per_wave * base_work * (double)waves)), (double)max_scale);
+}

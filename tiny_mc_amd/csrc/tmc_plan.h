@@ -115,6 +115,16 @@ struct Arena {
  * 1 and `max_per_lane` photons per lane, never more than `left` (left >= 1). */
 uint64_t chunk_photons(double work, double L, uint64_t max_per_lane, uint64_t left);
 
+/* The server's chunk sizing.  Its backlog counts each request's unpublished
+ * work (photons x L expected absorptions) up to `cap` (backlog_share), and
+ * the chunk work and per-lane cap are the base ones times
+ * serve_chunk_scale(backlog): 1 while the backlog is within
+ * chunks_per_wave base chunks per resident wave, the backlog over that
+ * amount beyond, at most max_scale (>= 1). */
+double backlog_share(uint64_t photons, double L, double cap);
+double serve_chunk_scale(double backlog, double base_work, uint32_t waves, double chunks_per_wave,
+                         uint32_t max_scale);
+
 } // namespace tmc
 
 #endif

@@ -211,7 +211,7 @@ uint64_t load_acquire(const uint64_t* p) { return __atomic_load_n(p, __ATOMIC_AC
  * absorptions each add to the backlog that sizes chunks. */
 double backlog_share(const tmc_server* s, uint64_t photons, double L)
 {
-    return std::min((double)photons * L, s->request_work_cap);
+    return tmc::backlog_share(photons, L, s->request_work_cap);
 }
 
 /* Writes ring entry s->published if its slot is free: the entry `ring`
@@ -373,9 +373,8 @@ void serve_step(tmc_server* s)
          * kChunksPerWave base chunks per wave, growing with it beyond (one
          * request alone to at most TMC_SERVE_REQUEST_SCALE times) */
         const double base_work = (double)s->opt.chunk_work;
-        const double scale = std::min<double>(
-            std::max(1.0, s->unpublished_work / (kChunksPerWave * base_work * (double)s->total_waves)),
-            (double)s->opt.max_chunk_scale);
+        const double scale = tmc::serve_chunk_scale(s->unpublished_work, base_work, s->total_waves, kChunksPerWave,
+                                                    s->opt.max_chunk_scale);
         const double work = base_work * scale;
         const uint64_t max_per_lane = (uint64_t)std::min<double>(
             (double)(1u << 24), std::floor((double)s->opt.max_photons_per_lane * scale));
